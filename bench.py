@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Fuse + raycast throughput on one MI355X.
+
+One step = one frame of the plain drop-in pipeline at one pose: block
+allocation, visibility + TSDF/weight/colour integration, raycast and vertex
+normals, on the room_vga workload (640x480 uint16-millimetre depth with
+colour, 5 mm voxels, 2 cm truncation, 2^20 buckets + 2^17 excess slots,
+2^20-block heap, float TSDF + float weight + uint8 colour, seeded random-walk
+trajectory). All frames are uploaded to HBM before the timed region.
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "frames/s fuse+raycast 640x480 @5mm, batched vs plain; % of HBM peak"
+
+# algorithmic bytes (per pixel / per visible block) of one frame step
+B_DEPTH, B_COLOR, B_VMAP, B_NMAP = 2, 3, 16, 16
+S_VOXEL, S_RAY, S_ENTRY = 11, 8, 16
+
+
+def stage_bytes(P, V):
+    """Algorithmic bytes per launch of each kernel for P pixels, V visible blocks."""
+    return {
+        "allocate": P * B_DEPTH,
+        "integrate": P * (B_DEPTH + B_COLOR) + V * (512 * 2 * S_VOXEL + S_ENTRY),
+        "raycast": V * 512 * S_RAY + P * B_VMAP,
+        "normals": P * (B_VMAP + B_NMAP),
+    }
+
+
+def frame_bytes(P, V):
+    return P * (B_DEPTH + B_COLOR + B_VMAP + B_NMAP) + V * (512 * (2 * S_VOXEL + S_RAY) + S_ENTRY)
+
+
+def cpu_baseline(w, seconds: float) -> dict:
+    """The oracle (OpenMP over blocks and pixels, serial allocation) on the
+    first frames of the same sequence, until `seconds` of work are done."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1))
+    O.set_threads(threads)
+    o = O.OracleMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, w.voxel_format)
+    frames = [w.frame(i) for i in range(8)]
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        d, c = frames[n % len(frames)]
+        T = w.poses(n + 1)[n]
+        o.fuse(w.depth_metres(d), w.K, T, rgb=c)
+        vm = np.zeros((w.height, w.width, 4), np.float32)
+        o.render_scene(vm, w.K, T)
+        O.compute_normals(vm)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 64:
+            break
+    return {"value": round(n / el, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"oracle fuse+raycast+normals of the first {n} frames of {w.name} "
+                      f"({el:.1f} s, OpenMP integration/raycast, serial allocation)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=480)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(scenes shard per GPU with no exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from voxelmapping_amd import _capi, synth
+    from voxelmapping_amd.mapping import DeviceMap, intrinsics, pose_colmajor
+
+    L = _capi.lib()
+    dev = torch.device("cuda:0")
+    nframes = args.warmup + args.steps
+    w = synth.room_vga(n_frames=max(500, nframes))
+    W, H = w.width, w.height
+    P = W * H
+
+    # inputs: generated on the host, resident in HBM before timing
+    depth_d, rgb_d = [], []
+    for i in range(nframes):
+        d, c = w.frame(i)
+        depth_d.append(torch.from_numpy(d.view(np.int16)).to(dev))
+        rgb_d.append(torch.from_numpy(c).to(dev))
+    poses = [pose_colmajor(w.poses(nframes)[i]) for i in range(nframes)]
+    K = intrinsics(w.K)
+    vmap = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
+    nmap = torch.zeros_like(vmap)
+    m = DeviceMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, w.voxel_format)
+    h = m.handle
+    stream = torch.cuda.current_stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    dpitch, rpitch, vpitch = W * 2, W * 3, W * 16
+    scale = C.c_float(w.depth_scale)
+    vptr, nptr = C.c_void_p(vmap.data_ptr()), C.c_void_p(nmap.data_ptr())
+
+    def step(i, ev=None):
+        dp, cp = C.c_void_p(depth_d[i].data_ptr()), C.c_void_p(rgb_d[i].data_ptr())
+        T = poses[i]
+        if ev: ev[0].record(stream)
+        _capi.check(L.vm_allocate_blocks(h, dp, dpitch, W, H, 1, scale, K, T, sp))
+        if ev: ev[1].record(stream)
+        _capi.check(L.vm_integrate(h, dp, dpitch, W, H, 1, scale, cp, rpitch, K, T, sp, None))
+        if ev: ev[2].record(stream)
+        _capi.check(L.vm_render_scene(h, vptr, vpitch, W, H, K, T, sp))
+        if ev: ev[3].record(stream)
+        _capi.check(L.vm_compute_normals(vptr, vpitch, nptr, vpitch, W, H, sp))
+        if ev: ev[4].record(stream)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    s0 = m.stats()
+
+    names = ["allocate", "integrate", "raycast", "normals"]
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(args.warmup + k, events[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s1 = m.stats()
+
+    stage_ms = {n: 0.0 for n in names}
+    for ev in events:
+        for j, n in enumerate(names):
+            stage_ms[n] += ev[j].elapsed_time(ev[j + 1])
+    stage_avg_us = {n: 1000.0 * stage_ms[n] / args.steps for n in names}
+    sum_vis = s1["sum_visible"] - s0["sum_visible"]
+    mean_vis = sum_vis / args.steps
+    total_bytes = args.steps * P * (B_DEPTH + B_COLOR + B_VMAP + B_NMAP) + sum_vis * (
+        512 * (2 * S_VOXEL + S_RAY) + S_ENTRY)
+    fps = args.steps / wall
+    per_launch = stage_bytes(P, mean_vis)
+    dom = max(names, key=lambda n: stage_avg_us[n])
+    ach = per_launch[dom] / (stage_avg_us[dom] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_launch": int(per_launch[dom]), "avg_launch_us": round(stage_avg_us[dom], 2)}
+    out = {
+        "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000.0 * wall / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "form": "plain", "width": W, "height": H, "voxel_size_m": w.voxel_size,
+                   "trunc_m": w.trunc, "n_bucket": w.n_bucket, "n_excess": w.n_excess, "n_block": w.n_block,
+                   "voxel_format": "f32_tsdf+f32_weight+rgb8", "depth": "uint16 mm", "colour": True,
+                   "frames": [args.warmup, args.warmup + args.steps]},
+        "roofline": roofline,
+        "frame_hbm_frac": round(total_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "mean_visible_blocks": round(mean_vis, 1),
+        "allocated_blocks": s1["allocated_blocks"],
+        "stage_avg_us": {n: round(v, 2) for n, v in stage_avg_us.items()},
+        "status": s1["status"],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(synth.room_vga(), args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

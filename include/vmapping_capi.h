@@ -1,0 +1,181 @@
+/*
+ * vmapping_capi.h -- C ABI of the MI355X (gfx950) voxel-hashing fusion
+ * pipeline. Plain pointers and sizes only; every image pointer is DEVICE
+ * memory (HBM), every pose/intrinsics array is HOST memory. All calls are
+ * asynchronous on `stream` (a hipStream_t, NULL = the null stream) unless an
+ * output count pointer is passed, which synchronises that stream.
+ *
+ * Return value: 0 on success, a negative VM_ERR_* code on failure; the
+ * message of the last failure on the calling thread is vm_last_error().
+ * Unlike the reference (which prints and calls exit(0) on any CUDA error,
+ * src/cuda_utils.h:16-26), errors are returned, never fatal.
+ *
+ * Conventions kept from the reference:
+ *  - camToWorld / worldToCam are column-major 4x4 float (Eigen::Matrix4f);
+ *  - K = {fx, fy, cx, cy} = {K(0,0), K(1,1), K(0,2), K(1,2)};
+ *  - depth is metres (float) or raw units (uint16) times depth_scale; 0, NaN
+ *    or values outside [depth_min, depth_max] are invalid;
+ *  - vertex maps are float4 per pixel in the camera frame, x = NaN on a miss;
+ *  - meshes are triangle soups: 9 floats per triangle, world metres.
+ */
+#ifndef VMAPPING_CAPI_H
+#define VMAPPING_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VM_OK 0
+#define VM_ERR_INVALID_ARG (-1)
+#define VM_ERR_HIP (-2)
+#define VM_ERR_OUT_OF_MEMORY (-3)
+#define VM_ERR_STATE (-4)
+
+/* voxel storage formats */
+#define VM_VOXEL_I16_U8 0     /* vmap::Voxel    {short sdf; uchar wt}            4 B */
+#define VM_VOXEL_I16_U8_RGB 1 /* vmap::VoxelRGB {short sdf; uchar wt; uchar[3]}  6 B */
+#define VM_VOXEL_F32_F32_RGB8 2 /* {float sdf; float wt; uchar rgb[3]; pad}     12 B */
+
+/* depth image formats */
+#define VM_DEPTH_F32 0 /* float metres (the reference's GpuMat CV_32FC1) */
+#define VM_DEPTH_U16 1 /* uint16 raw units, metres = raw * depth_scale */
+
+/* status bits (vm_map_stats.status); sticky until vm_map_reset */
+#define VM_STATUS_HEAP_FULL 1
+#define VM_STATUS_EXCESS_FULL 2
+#define VM_STATUS_CHAIN_BOUND 4
+#define VM_STATUS_MESH_CLAMPED 8
+
+typedef struct vm_map vm_map;
+
+/* Map geometry. Replaces MapStruct::create(nEntry, nBucket, nVBlock,
+ * voxelSize, truncationDist), include/map_struct.h:50 / src/map_struct.cu:44-66,
+ * plus the compile-time MIN_DEPTH / MAX_DEPTH of include/map_struct.h:9-10. */
+typedef struct vm_map_desc {
+    int32_t n_entry;   /* hash slots: n_bucket heads + excess list */
+    int32_t n_bucket;  /* primary buckets, 0 < n_bucket <= n_entry */
+    int32_t n_block;   /* voxel-block heap capacity, <= 2^22 */
+    float voxel_size;  /* metres */
+    float trunc_dist;  /* metres */
+    float depth_min;   /* 0.1 in the reference */
+    float depth_max;   /* 5.0 in the reference */
+    int32_t voxel_format;
+    float max_weight;  /* weight cap of VM_VOXEL_F32_F32_RGB8 (255 by default);
+                          the uchar formats saturate at 255 */
+} vm_map_desc;
+
+/* Device arrays of a map, laid out as the reference's MapStruct<TVoxel>
+ * members (include/map_struct.h:54-67). Owned by the map. */
+typedef struct vm_map_arrays {
+    void* hash_table;     /* HashEntry[n_entry]  {int ptr, offset, x, y, z} */
+    void* visible_table;  /* HashEntry[n_entry]  compacted visible entries */
+    int32_t* heap;        /* int[n_block] free-block stack */
+    int32_t* heap_ptr;    /* int */
+    int32_t* excess_ptr;  /* int */
+    int32_t* bucket_mutex;/* int[n_bucket] */
+    uint32_t* visible_block_num; /* uint */
+    void* voxel_block;    /* voxels[n_block * 512] */
+    int32_t* block_pos;   /* int4[n_block] {x, y, z, slot} of allocated blocks */
+} vm_map_arrays;
+
+typedef struct vm_map_stats {
+    int32_t allocated_blocks; /* n_block - 1 - heap_ptr */
+    int32_t heap_ptr;
+    int32_t excess_used;      /* excess_ptr */
+    int32_t visible_blocks;   /* last visibility pass */
+    int32_t status;           /* VM_STATUS_* bits */
+    int32_t frames_fused;
+    int64_t sum_visible;      /* sum of visible counts over fused frames */
+} vm_map_stats;
+
+const char* vm_last_error(void);
+/* library version string */
+const char* vm_version(void);
+
+/* MapStruct::create + reset (map_struct.cu:29-66). */
+int vm_map_create(const vm_map_desc* desc, vm_map** out_map);
+/* MapStruct::release (map_struct.cu:68-79). NULL is accepted. */
+int vm_map_destroy(vm_map* map);
+/* MapStruct::reset (map_struct.cu:29-42): empty table, full heap, voxels 0. */
+int vm_map_reset(vm_map* map, void* stream);
+int vm_map_get_desc(const vm_map* map, vm_map_desc* out_desc);
+int vm_map_get_arrays(const vm_map* map, vm_map_arrays* out_arrays);
+/* Counters; synchronises `stream`. getVisibleBlockCount (map_struct.cu:81-85)
+ * is stats.visible_blocks. */
+int vm_map_get_stats(vm_map* map, void* stream, vm_map_stats* out_stats);
+
+/* FuseImage<TVoxel> (depth_fusion.cu:199-301) and, with rgb != NULL,
+ * FuseDepthAndImage<TVoxel> (depth_fusion.cu:383-488): allocate the blocks
+ * traced by every valid pixel, compact the visible blocks, integrate TSDF,
+ * weight (and colour). rgb: 3 bytes per pixel, device, pitch rgb_pitch; it is
+ * ignored by the VM_VOXEL_I16_U8 format. If out_visible_count is not NULL the
+ * stream is synchronised and the visible-block count returned there (the
+ * reference's return value). */
+int vm_fuse_depth(vm_map* map, const void* depth, int64_t depth_pitch, int32_t width, int32_t height,
+                  int32_t depth_format, float depth_scale, const void* rgb, int64_t rgb_pitch,
+                  const float K[4], const float cam_to_world[16], void* stream, uint32_t* out_visible_count);
+
+/* Stage API, as the reference's internal device_functions.h split: */
+/* CreateBlockLineTracingFunctor launch alone (depth_fusion.cu:214-236). */
+int vm_allocate_blocks(vm_map* map, const void* depth, int64_t depth_pitch, int32_t width, int32_t height,
+                       int32_t depth_format, float depth_scale, const float K[4], const float cam_to_world[16],
+                       void* stream);
+/* The rest of FuseImage after allocation (CheckEntryVisibilityFunctor +
+ * DepthFusionFunctor / ImageFusionFunctor, depth_fusion.cu:241-298, :427-484)
+ * as one fused pass. vm_allocate_blocks followed by vm_integrate equals
+ * vm_fuse_depth. */
+int vm_integrate(vm_map* map, const void* depth, int64_t depth_pitch, int32_t width, int32_t height,
+                 int32_t depth_format, float depth_scale, const void* rgb, int64_t rgb_pitch, const float K[4],
+                 const float cam_to_world[16], void* stream, uint32_t* out_visible_count);
+/* CheckVisibleBlocks<TVoxel> (depth_fusion.cu:494-539). */
+int vm_check_visible_blocks(vm_map* map, int32_t width, int32_t height, const float K[4],
+                            const float cam_to_world[16], void* stream, uint32_t* out_visible_count);
+
+/* RenderScene<TVoxel> (ray_tracing.cu:403-437): raycast every pixel into
+ * vmap (float4, camera frame, x = NaN on a miss). */
+int vm_raycast(vm_map* map, float* vmap, int64_t vmap_pitch, int32_t width, int32_t height, const float K[4],
+               const float cam_to_world[16], void* stream);
+/* VoxelMapping::RenderScene (vmapping.cpp:83-104): visible-block check, then
+ * the raycast unless nothing is visible (vmap is then left untouched). The
+ * visibility pass is skipped when the map has not changed since a pass at the
+ * same pose, size and intrinsics (the result would be identical). */
+int vm_render_scene(vm_map* map, float* vmap, int64_t vmap_pitch, int32_t width, int32_t height,
+                    const float K[4], const float cam_to_world[16], void* stream);
+/* ProjectRenderingBlocks<TVoxel> (ray_tracing.cu:171-234) over the current
+ * visible table: z-range maps at 1/8 resolution (cols8 x rows8 floats each,
+ * pitch in bytes). world_to_cam as the reference passes it. */
+int vm_project_rendering_blocks(vm_map* map, float* zrange_x, float* zrange_y, int64_t pitch, int32_t cols8,
+                                int32_t rows8, const float K[4], const float world_to_cam[16], void* stream,
+                                uint32_t* out_rendering_block_count);
+
+/* vmap::computeNormal (image_proc.cu:95-104). The last row and column are
+ * not written, as in the reference. */
+int vm_compute_normals(const float* vmap, int64_t vmap_pitch, float* nmap, int64_t nmap_pitch, int32_t width,
+                       int32_t height, void* stream);
+/* vmap::renderScene (image_proc.cu:63-72): Phong shading to RGBA8. */
+int vm_render_shading(const float* vmap, int64_t vmap_pitch, const float* nmap, int64_t nmap_pitch,
+                      uint8_t* rgba, int64_t rgba_pitch, int32_t width, int32_t height, void* stream);
+
+/* Polygonize<TVoxel> (marching_cubes.cu:251-305): triangle soup of every
+ * allocated block into verts / norms (device, cap_triangles * 9 floats each).
+ * Synchronises; returns the triangle count clamped to cap_triangles and the
+ * number of allocated blocks. */
+int vm_polygonize(vm_map* map, float* verts, float* norms, int64_t cap_triangles, void* stream,
+                  uint32_t* out_block_count, uint32_t* out_triangle_count);
+/* GetSurfacePoints (marching_cubes.cu:330-344): triangle centroids. */
+int vm_surface_points(const float* verts, float* points, int64_t n_triangles, void* stream);
+
+/* State transfer for tests and tools: copy a whole map array to/from HOST
+ * memory. which: 0 hash table, 1 heap, 2 voxels, 3 counters (16 ints),
+ * 4 visible table, 5 block positions. vm_map_upload rebuilds the block
+ * index from the hash table after uploading the hash table. Synchronous. */
+int vm_map_download(vm_map* map, int32_t which, void* host_dst, int64_t bytes);
+int vm_map_upload(vm_map* map, int32_t which, const void* host_src, int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

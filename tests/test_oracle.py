@@ -1,0 +1,453 @@
+"""Pins of the CPU oracle (oracle/vm_oracle.cpp) against facts that do not
+come from the oracle itself: hand-computed known answers, brute force over
+integer ranges, closed forms of simple scenes, numpy's matrix inverse, the
+marching-cubes table invariants and the reference's table text.
+
+Pinned here, per oracle function:
+  hash3                 hand-computed KATs + range invariant
+  voxel_to_block/local  brute force vs Python floor division / modulo
+  f2i_rd                special values
+  pose_inverse          numpy.linalg.inv (float64) and rigid closed form
+  MC tables             820 triangles, edge masks, reference text (if present)
+  allocate_pixels +
+  create_block          hand-derived block sets (1-pixel images), capacity
+                        exhaustion, and the excess-slot alias of the reference
+  check_visible +
+  integrate             fronto-parallel plane closed form, n-frame weights,
+                        int16 packing
+  raycast               plane closed form (hit depth <= 1e-4 m)
+  compute_normals       plane closed form (normal (0, 0, 1))
+  polygonize            all 256 single-cube cases vs the table, sphere mesh
+                        (vertex radius, closed surface, consistent winding)
+  surface_points        centroid closed form
+Parity unpinned: project_rendering_blocks and render_shading (visual
+diagnostics outside the measured path) are checked only for basic invariants.
+"""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+REF_TABLE = "/root/reference/src/triangle_table.h"
+
+
+def colmajor(T):
+    return np.asarray(T, np.float32).T.reshape(-1)
+
+
+def c_int32(v):
+    v &= 0xFFFFFFFF
+    return v - (1 << 32) if v >= (1 << 31) else v
+
+
+# --- hash ----------------------------------------------------------------------
+
+def test_hash_known_answers():
+    n = 1 << 20
+    assert O.hash_block(1, 0, 0, n) == 455773        # 73856093 mod 2^20
+    assert O.hash_block(0, 1, 0, n) == 475301        # 19349669 mod 2^20
+    assert O.hash_block(0, 0, 1, n) == 655287        # 83492791 mod 2^20
+    assert O.hash_block(-1, 0, 0, n) == 592803       # 2^20 - 455773
+    assert O.hash_block(100, 0, 0, n) == 488532      # 7385609300 wraps to -1204325292
+    assert O.hash_block(0, 0, 0, 943718) == 0
+    # non power of two: 73856093 = 78 * 943718 + 246089
+    assert O.hash_block(1, 0, 0, 943718) == 246089
+
+
+def test_hash_range_and_wrap():
+    rng = np.random.default_rng(0)
+    for n in (1, 7, 1 << 17, 943718, (1 << 20) + 3):
+        for _ in range(200):
+            x, y, z = (int(v) for v in rng.integers(-(1 << 28), 1 << 28, 3))
+            h = O.hash_block(x, y, z, n)
+            assert 0 <= h < n
+            r = c_int32(x * 73856093) ^ c_int32(y * 19349669) ^ c_int32(z * 83492791)
+            assert h == r - n * int(r / n) + (n if r - n * int(r / n) < 0 else 0)
+
+
+# --- coordinates -----------------------------------------------------------------
+
+def test_voxel_block_local_bruteforce():
+    for v in range(-64, 65):
+        b = O.voxel_to_block((v, -v, v // 3))
+        assert list(b) == [v // 8, (-v) // 8, (v // 3) // 8]
+        l = O.voxel_to_local((v, -v, v // 3))
+        assert list(l) == [v % 8, (-v) % 8, (v // 3) % 8]
+
+
+def test_float2int_rd_special_values():
+    f = O.lib().or_float2int_rd
+    assert f(-0.5) == -1 and f(2.999) == 2 and f(-3.0) == -3 and f(0.0) == 0
+    assert f(float("nan")) == 0
+    assert f(1e10) == 2**31 - 1 and f(-1e10) == -(2**31)
+
+
+# --- pose inverse ----------------------------------------------------------------
+
+def random_pose(rng, scale=3.0):
+    a = rng.normal(size=3)
+    th = rng.uniform(0, math.pi)
+    k = a / np.linalg.norm(a)
+    Kx = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    R = np.eye(3) + math.sin(th) * Kx + (1 - math.cos(th)) * Kx @ Kx
+    T = np.eye(4)
+    T[:3, :3] = R
+    T[:3, 3] = rng.uniform(-scale, scale, 3)
+    return T.astype(np.float32)
+
+
+def test_pose_inverse_matches_numpy():
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        T = random_pose(rng)
+        inv = O.pose_inverse(colmajor(T)).reshape(4, 4).T
+        ref = np.linalg.inv(T.astype(np.float64))
+        assert np.allclose(inv, ref, atol=2e-7, rtol=1e-6)
+        # rigid closed form [R^T | -R^T t]
+        R, t = T[:3, :3].astype(np.float64), T[:3, 3].astype(np.float64)
+        assert np.allclose(inv[:3, :3], R.T, atol=1e-6)
+        assert np.allclose(inv[:3, 3], -R.T @ t, atol=1e-5)
+
+
+# --- marching-cubes tables -------------------------------------------------------
+
+CORNERS = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+
+
+def test_mc_table_invariants():
+    edge, tri = O.mc_tables()
+    counts = [(row != -1).sum() for row in tri]
+    assert sum(counts) == 820 * 3
+    assert max(counts) == 15 and all(c % 3 == 0 for c in counts)
+    for c in range(256):
+        assert edge[c] == edge[255 - c]
+        used = 0
+        for e in tri[c][tri[c] != -1]:
+            used |= 1 << int(e)
+        assert used == edge[c], c
+        cut = sum(1 << k for k, (a, b) in enumerate(CORNERS) if ((c >> a) & 1) != ((c >> b) & 1))
+        assert cut == edge[c]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TABLE), reason="reference tree not mounted")
+def test_mc_table_equals_reference_text():
+    edge, tri = O.mc_tables()
+    text = open(REF_TABLE).read()
+    etext = text[text.index("edgeTable"):text.index("triTable")]
+    ref_edge = [int(v, 16) for v in re.findall(r"0x[0-9a-fA-F]+", etext)]
+    assert ref_edge == edge.tolist()
+    rows = re.findall(r"\{([-0-9, ]+)\}", text[text.index("triTable"):])
+    ref_tri = [[int(v) for v in r.split(",")] for r in rows]
+    assert ref_tri == tri.tolist()
+
+
+# --- allocation: hand-derived block sets -------------------------------------------
+
+K1 = np.array([1.0, 1.0, 0.0, 0.0], np.float32)  # 1x1 image, pixel (0,0) on the optical axis
+
+
+def one_pixel_map(n_bucket=64, n_excess=16, n_block=64, fmt=0):
+    return O.OracleMap(n_bucket + n_excess, n_bucket, n_block, 0.01, 0.04, fmt)
+
+
+def allocated(m):
+    return set(O.allocated_blocks(m.hash_table()).keys())
+
+
+def test_allocation_single_block():
+    m = one_pixel_map()
+    # band [0.983, 1.023] m -> voxels z 98..102 -> block 12 (voxels 96..103)
+    m.allocate(np.array([[1.003]], np.float32), K1, colmajor(np.eye(4)))
+    assert allocated(m) == {(0, 0, 12)}
+
+
+def test_allocation_crosses_block_boundary():
+    m = one_pixel_map()
+    # band [1.021, 1.061] -> voxels 102..106 -> blocks 12 and 13
+    m.allocate(np.array([[1.041]], np.float32), K1, colmajor(np.eye(4)))
+    assert allocated(m) == {(0, 0, 12), (0, 0, 13)}
+    c = m.counters()
+    assert c["heap_ptr"] == 64 - 1 - 2 and c["alias_events"] == 0
+    # first pop is block 0 (heap[i] = n - 1 - i), and the nearer block is traced first
+    tab = m.hash_table()
+    ptr = {tuple(e[2:]): e[0] for e in tab if e[0] >= 0}
+    assert ptr[(0, 0, 12)] == 0 and ptr[(0, 0, 13)] == 512
+
+
+def test_allocation_negative_coordinates():
+    m = one_pixel_map()
+    flip = np.diag([1.0, -1.0, -1.0, 1.0])  # camera looks down world -z
+    m.allocate(np.array([[1.041]], np.float32), K1, colmajor(flip))
+    # voxels -103 .. -107 (samples truncate toward zero) -> blocks -13, -14
+    assert allocated(m) == {(0, 0, -13), (0, 0, -14)}
+
+
+def test_allocation_invalid_depths_allocate_nothing():
+    m = one_pixel_map()
+    for d in (0.0, float("nan"), 0.05, 5.5):
+        m.allocate(np.array([[d]], np.float32), K1, colmajor(np.eye(4)))
+    assert allocated(m) == set()
+
+
+def test_allocation_heap_exhaustion():
+    m = one_pixel_map(n_block=1)
+    m.allocate(np.array([[1.041]], np.float32), K1, colmajor(np.eye(4)))
+    assert allocated(m) == {(0, 0, 12)}
+    c = m.counters()
+    assert c["heap_ptr"] == -1 and c["alloc_failures"] >= 1
+
+
+def test_excess_alias_quirk_of_reference():
+    # one bucket: the second key's excess offset 0 maps to slot nBucket-1 = 0,
+    # i.e. onto the occupied head (map_struct.h:344); the reference overwrites it
+    m = O.OracleMap(3, 1, 8, 0.01, 0.04, 0)
+    m.allocate(np.array([[1.041]], np.float32), K1, colmajor(np.eye(4)))
+    c = m.counters()
+    assert c["alias_events"] == 1
+    tab = m.hash_table()
+    assert tuple(tab[0][2:]) == (0, 0, 13) and tab[0][1] == 0  # self-linked head
+    assert allocated(m) == {(0, 0, 13)}
+
+
+# --- integration: plane closed form ----------------------------------------------------
+
+W, H = 32, 24
+KP = np.array([30.0, 30.0, 15.5, 11.5], np.float32)
+D = 1.0037
+VS, TR = 0.01, 0.04
+
+
+def plane_map(fmt=2, frames=1):
+    m = O.OracleMap(4096 + 1024, 4096, 2048, VS, TR, fmt)
+    depth = np.full((H, W), D, np.float32)
+    for _ in range(frames):
+        m.fuse(depth, KP, colmajor(np.eye(4)))
+    return m
+
+
+def voxel_coords(table, n_block):
+    """(vx, vy, vz, voxel index) of every voxel of every allocated block."""
+    out = []
+    lz, ly, lx = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    local = (lz * 64 + ly * 8 + lx).reshape(-1)
+    for e in table[table[:, 0] >= 0]:
+        ptr, bx, by, bz = e[0], e[2], e[3], e[4]
+        out.append(np.stack([bx * 8 + lx.reshape(-1), by * 8 + ly.reshape(-1), bz * 8 + lz.reshape(-1),
+                             ptr + local], 1))
+    return np.concatenate(out)
+
+
+def expected_plane(vc):
+    x, y, z = (vc[:, i].astype(np.float64) * VS for i in range(3))
+    u = KP[0] * x / z + KP[2] + 0.5
+    v = KP[1] * y / z + KP[3] + 0.5
+    inside = (u >= 0) & (v >= 0) & (u < W) & (v < H)
+    border = (np.abs(u - np.round(u)) < 1e-3) & ((np.round(u) == 0) | (np.round(u) == W))
+    border |= (np.abs(v - np.round(v)) < 1e-3) & ((np.round(v) == 0) | (np.round(v) == H))
+    sd = D - z
+    upd = inside & (sd >= -TR)
+    near_edge = border | (np.abs(sd + TR) < 1e-6)
+    return upd, np.minimum(1.0, sd / TR), near_edge
+
+
+def test_integration_plane_first_frame():
+    m = plane_map(fmt=2)
+    tab = m.hash_table()
+    vis = {tuple(e[2:]) for e in m.visible_table()}
+    assert vis == set(O.allocated_blocks(tab).keys())  # every allocated block is in view
+    vc = voxel_coords(tab, m.n_block)
+    sdf, w, _ = O.decode_voxels(m.voxels(), 2)
+    upd, exp, edge = expected_plane(vc)
+    s, ww = sdf[vc[:, 3]], w[vc[:, 3]]
+    ok = ~edge
+    assert np.array_equal(ww[ok] == 1.0, upd[ok])
+    assert np.all(ww[ok & ~upd] == 0)
+    assert np.max(np.abs(s[ok & upd] - exp[ok & upd])) < 2e-6
+    assert upd.sum() > 1000
+
+
+def test_integration_plane_weights_accumulate():
+    m = plane_map(fmt=2, frames=7)
+    tab = m.hash_table()
+    vc = voxel_coords(tab, m.n_block)
+    sdf, w, _ = O.decode_voxels(m.voxels(), 2)
+    upd, exp, edge = expected_plane(vc)
+    ok = ~edge & upd
+    assert np.all(w[vc[ok, 3]] == 7.0)
+    assert np.max(np.abs(sdf[vc[ok, 3]] - exp[ok])) < 4e-6
+
+
+def test_integration_plane_int16_packing():
+    m = plane_map(fmt=0, frames=3)
+    tab = m.hash_table()
+    vc = voxel_coords(tab, m.n_block)
+    sdf, w, _ = O.decode_voxels(m.voxels(), 0)
+    upd, exp, edge = expected_plane(vc)
+    ok = ~edge & upd
+    assert np.all(w[vc[ok, 3]] == 3)
+    # packFloat truncates: stored = trunc(v * 32767) / 32767, repeated 3 times
+    assert np.max(np.abs(sdf[vc[ok, 3]] - exp[ok])) <= 3.0 / 32767 + 1e-6
+
+
+def test_weight_saturates_at_255():
+    m = O.OracleMap(4096 + 1024, 4096, 2048, VS, TR, 0)
+    depth = np.full((4, 4), D, np.float32)
+    k = np.array([3.0, 3.0, 1.5, 1.5], np.float32)
+    for _ in range(260):
+        m.fuse(depth, k, colmajor(np.eye(4)))
+    _, w, _ = O.decode_voxels(m.voxels(), 0)
+    assert w.max() == 255
+
+
+def test_colour_running_average():
+    m = O.OracleMap(4096 + 1024, 4096, 2048, VS, TR, 1)
+    depth = np.full((H, W), D, np.float32)
+    c1 = np.full((H, W, 3), 200, np.uint8)
+    c2 = np.full((H, W, 3), 100, np.uint8)
+    m.fuse(depth, KP, colmajor(np.eye(4)), rgb=c1)
+    m.fuse(depth, KP, colmajor(np.eye(4)), rgb=c2)
+    _, w, rgb = O.decode_voxels(m.voxels(), 1)
+    sel = w == 2
+    assert sel.sum() > 0
+    # first observation copies, then u8(0.7*old + 0.3*new) = u8(170.0)
+    exp = int(np.float32(0.7) * np.float32(200) + np.float32(0.3) * np.float32(100))
+    assert np.all(rgb[sel] == exp)
+
+
+# --- raycast / normals: plane closed form ----------------------------------------------
+
+def test_raycast_plane_depth():
+    m = plane_map(fmt=2)
+    v = m.raycast(W, H, KP, colmajor(np.eye(4)))
+    ys, xs = np.mgrid[0:H, 0:W]
+    dz = 1.0 / np.sqrt(((xs - KP[2]) / KP[0]) ** 2 + ((ys - KP[3]) / KP[1]) ** 2 + 1.0)
+    sel = (dz > 0.985) & (xs > 0) & (ys > 0) & (xs < W - 1) & (ys < H - 1)
+    assert sel.sum() > 50
+    hit = ~np.isnan(v[..., 0])
+    assert hit[sel].all()
+    assert np.max(np.abs(v[..., 2][sel] - D)) < 1e-4
+    assert np.all(v[..., 3][hit] == 1.0)
+
+
+def test_normals_plane():
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    z = np.float32(D)
+    vmap = np.stack([(xs - KP[2]) / KP[0] * z, (ys - KP[3]) / KP[1] * z, np.full_like(xs, z),
+                     np.ones_like(xs)], -1).astype(np.float32)
+    n = O.compute_normals(vmap)
+    inner = n[:H - 1, :W - 1]
+    assert np.allclose(inner[..., :3], [0, 0, 1], atol=1e-6)
+    assert np.all(inner[..., 3] == 1)
+    assert np.all(n[H - 1] == 0) and np.all(n[:, W - 1] == 0)
+
+
+def test_surface_points_centroid():
+    v = np.arange(18, dtype=np.float32).reshape(2, 9)
+    p = O.surface_points(v)
+    assert np.allclose(p, v.reshape(2, 3, 3).mean(1))
+
+
+# --- marching cubes ----------------------------------------------------------------
+
+OFFS = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+EDGE_MID = {k: tuple((np.array(OFFS[a]) + np.array(OFFS[b])) / 2.0) for k, (a, b) in enumerate(CORNERS)}
+
+
+def set_voxel(raw, fmt, idx, sdf, w):
+    if fmt == 2:
+        rec = raw.reshape(-1, 12)
+        rec[idx, 0:4] = np.frombuffer(np.float32(sdf).tobytes(), np.uint8)
+        rec[idx, 4:8] = np.frombuffer(np.float32(w).tobytes(), np.uint8)
+    else:
+        rec = raw.reshape(-1, 4)
+        rec[idx, 0:2] = np.frombuffer(np.int16(int(sdf * 32767)).tobytes(), np.uint8)
+        rec[idx, 2] = w
+
+
+def test_polygonize_single_cube_cases():
+    _, tri = O.mc_tables()
+    for case in range(256):
+        m = O.OracleMap(64 + 16, 64, 8, 1.0, 4.0, 2)
+        ptr = m.insert_block(0, 0, 0)
+        raw = m.voxels_view()
+        for i, (dx, dy, dz) in enumerate(OFFS):
+            set_voxel(raw, 2, ptr + dz * 64 + dy * 8 + dx, -0.5 if (case >> i) & 1 else 0.5, 1.0)
+        _, n, v, _ = m.polygonize(16)
+        row = [e for e in tri[case] if e != -1]
+        assert n == len(row) // 3, case
+        for t in range(n):
+            a, b, c = row[3 * t:3 * t + 3]
+            got = v[t].reshape(3, 3)
+            assert np.allclose(got, [EDGE_MID[c], EDGE_MID[b], EDGE_MID[a]]), case
+
+
+def sphere_map(r=0.1, c=(0.3037, 0.2961, 0.3013), vs=0.01, tr=0.04):
+    m = O.OracleMap(4096 + 1024, 4096, 512, vs, tr, 2)
+    raw = m.voxels_view()
+    lo = int(math.floor((c[0] - r - 2 * tr) / vs)) // 8
+    hi = int(math.floor((c[0] + r + 2 * tr) / vs)) // 8
+    lz, ly, lx = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    rec = raw.reshape(-1, 12)
+    for bx in range(lo, hi + 1):
+        for by in range(lo, hi + 1):
+            for bz in range(lo, hi + 1):
+                ptr = m.insert_block(bx, by, bz)
+                p = np.stack([(bx * 8 + lx) * vs, (by * 8 + ly) * vs, (bz * 8 + lz) * vs], -1).reshape(-1, 3)
+                d = np.sqrt(((p - np.array(c)) ** 2).sum(1)) - r
+                s = np.clip(d / tr, -1, 1).astype(np.float32)
+                idx = ptr + (lz * 64 + ly * 8 + lx).reshape(-1)
+                rec[idx, 0:4] = s.view(np.uint8).reshape(-1, 4)
+                rec[idx, 4:8] = np.frombuffer(np.float32(1).tobytes(), np.uint8)
+    return m
+
+
+def merged_vertex_ids(pts, tol=1e-6):
+    """Vertex ids after merging points closer than tol (the same edge crossing
+    computed from two neighbouring cubes can differ in the last bit)."""
+    from scipy.spatial import cKDTree
+    parent = np.arange(len(pts))
+
+    def find(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+
+    for a, b in cKDTree(pts).query_pairs(tol):
+        ra, rb = find(a), find(b)
+        if ra != rb:
+            parent[max(ra, rb)] = min(ra, rb)
+    return np.array([find(i) for i in range(len(pts))])
+
+
+def test_polygonize_sphere():
+    c, r = np.array([0.3037, 0.2961, 0.3013]), 0.1
+    m = sphere_map()
+    blocks, n, v, nrm = m.polygonize(200000)
+    assert n > 1000 and blocks == 6 ** 3  # blocks 1..6 per axis cover [0.12, 0.48] m
+    pts = v.reshape(-1, 3)
+    assert np.max(np.abs(np.linalg.norm(pts - c, axis=1) - r)) < 5e-4
+    # closed surface: every undirected edge is shared by an even number of
+    # triangles (2, or 4 where a corner value is ~0 and vertices coincide)
+    q = merged_vertex_ids(pts).reshape(-1, 3)
+    edges = {}
+    for t in q:
+        for i in range(3):
+            a, b = int(t[i]), int(t[(i + 1) % 3])
+            if a == b:
+                continue
+            key = (a, b) if a < b else (b, a)
+            edges[key] = edges.get(key, 0) + 1
+    counts = np.array(list(edges.values()))
+    assert np.all(counts % 2 == 0)
+    assert np.mean(counts == 2) > 0.99
+    # consistent winding: face normals all point the same way relative to the centre
+    cen = v.reshape(-1, 3, 3).mean(1)
+    s = np.sign(((cen - c) * nrm[:, :3]).sum(1))
+    assert abs(s.mean()) > 0.99
+    # surface points are the centroids
+    assert np.allclose(O.surface_points(v), cen, atol=1e-7)

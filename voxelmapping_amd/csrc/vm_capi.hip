@@ -1,0 +1,503 @@
+// C ABI of the gfx950 fusion pipeline (declared in include/vmapping_capi.h).
+// Owns map allocation, argument checking, host-side constants (intrinsic
+// reciprocals, pose inverse) and stage sequencing. No host synchronisation
+// happens unless the caller asks for a count.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "vm_common.h"
+#include "vm_launch.h"
+#include "vmapping_capi.h"
+
+using namespace vmamd;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return VM_ERR_HIP;
+}
+
+#define VM_HIP(call, what)                      \
+    do {                                        \
+        hipError_t e_ = (call);                 \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+Affine affine_of(const float* T) {
+    Affine a;
+    a.r00 = T[0]; a.r01 = T[4]; a.r02 = T[8];  a.t0 = T[12];
+    a.r10 = T[1]; a.r11 = T[5]; a.r12 = T[9];  a.t1 = T[13];
+    a.r20 = T[2]; a.r21 = T[6]; a.r22 = T[10]; a.t2 = T[14];
+    return a;
+}
+
+// General 4x4 inverse by cofactors in double, rounded to float. The
+// reference uses Eigen's Matrix4f::inverse() (depth_fusion.cu:251,
+// ray_tracing.cu:414); this is the correctly rounded form of the same matrix.
+bool inverse4(const float* Tf, float* out) {
+    double m[16];
+    for (int i = 0; i < 16; ++i) m[i] = Tf[i];
+    auto A = [&](int r, int c) { return m[c * 4 + r]; };
+    double cof[4][4];
+    for (int r = 0; r < 4; ++r) {
+        for (int c = 0; c < 4; ++c) {
+            int rr[3], cc[3], k = 0, l = 0;
+            for (int i = 0; i < 4; ++i) if (i != r) rr[k++] = i;
+            for (int j = 0; j < 4; ++j) if (j != c) cc[l++] = j;
+            double d = A(rr[0], cc[0]) * (A(rr[1], cc[1]) * A(rr[2], cc[2]) - A(rr[1], cc[2]) * A(rr[2], cc[1])) -
+                       A(rr[0], cc[1]) * (A(rr[1], cc[0]) * A(rr[2], cc[2]) - A(rr[1], cc[2]) * A(rr[2], cc[0])) +
+                       A(rr[0], cc[2]) * (A(rr[1], cc[0]) * A(rr[2], cc[1]) - A(rr[1], cc[1]) * A(rr[2], cc[0]));
+            cof[r][c] = ((r + c) & 1) ? -d : d;
+        }
+    }
+    double det = 0;
+    for (int c = 0; c < 4; ++c) det += A(0, c) * cof[0][c];
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[c * 4 + r] = (float)(cof[c][r] / det);
+    return true;
+}
+
+Intrinsics intrinsics_of(const float* K) {
+    Intrinsics k;
+    k.fx = K[0]; k.fy = K[1]; k.cx = K[2]; k.cy = K[3];
+    k.ifx = (float)(1.0 / (double)K[0]);
+    k.ify = (float)(1.0 / (double)K[1]);
+    return k;
+}
+
+bool finite_all(const float* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+struct vm_map {
+    vm_map_desc desc;
+    MapView view;
+    int device;
+    // visibility reuse: the visible table is a pure function of the allocated
+    // set and (size, K, pose); `epoch` counts allocated-set changes
+    uint64_t epoch = 0;
+    bool visValid = false;
+    uint64_t visEpoch = 0;
+    int visW = 0, visH = 0;
+    float visK[4];
+    float visT[16];
+    // set by an allocation launch, which zeroes the visible count on the device
+    bool visCountZeroed = false;
+};
+
+namespace {
+
+int check_map(const vm_map* m) {
+    if (!m) return fail(VM_ERR_INVALID_ARG, "map is NULL");
+    return VM_OK;
+}
+
+int check_pose_k(const float* K, const float* T) {
+    if (!K || !T) return fail(VM_ERR_INVALID_ARG, "K and pose must not be NULL");
+    if (!finite_all(K, 4) || !finite_all(T, 16)) return fail(VM_ERR_INVALID_ARG, "K and pose must be finite");
+    if (K[0] == 0.f || K[1] == 0.f) return fail(VM_ERR_INVALID_ARG, "focal lengths must be non-zero");
+    return VM_OK;
+}
+
+int check_image(const void* p, int64_t pitch, int w, int h, int bpp, const char* name) {
+    if (w <= 0 || h <= 0) return fail(VM_ERR_INVALID_ARG, std::string(name) + ": width and height must be > 0");
+    if (w > 32767 || h > 32767) return fail(VM_ERR_INVALID_ARG, std::string(name) + ": image larger than 32767");
+    if (!p) return fail(VM_ERR_INVALID_ARG, std::string(name) + " is NULL");
+    if (pitch < (int64_t)w * bpp) return fail(VM_ERR_INVALID_ARG, std::string(name) + ": pitch smaller than a row");
+    return VM_OK;
+}
+
+void remember_visibility(vm_map* m, int W, int H, const float* K, const float* T) {
+    m->visCountZeroed = false;  // a visibility pass has counted into it
+    m->visValid = true;
+    m->visEpoch = m->epoch;
+    m->visW = W;
+    m->visH = H;
+    std::memcpy(m->visK, K, sizeof(m->visK));
+    std::memcpy(m->visT, T, sizeof(m->visT));
+}
+
+bool visibility_current(const vm_map* m, int W, int H, const float* K, const float* T) {
+    return m->visValid && m->visEpoch == m->epoch && m->visW == W && m->visH == H &&
+           std::memcmp(m->visK, K, sizeof(m->visK)) == 0 && std::memcmp(m->visT, T, sizeof(m->visT)) == 0;
+}
+
+int read_counter(vm_map* m, int idx, hipStream_t s, int* out) {
+    VM_HIP(hipMemcpyAsync(out, m->view.ctr + idx, sizeof(int), hipMemcpyDeviceToHost, s), "counter copy");
+    VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    return VM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vm_last_error(void) { return g_error.c_str(); }
+const char* vm_version(void) { return "vmapping-amd 0.1 gfx950"; }
+
+int vm_map_create(const vm_map_desc* d, vm_map** out) {
+    if (!d || !out) return fail(VM_ERR_INVALID_ARG, "desc and out_map must not be NULL");
+    *out = nullptr;
+    if (d->n_bucket <= 0 || d->n_entry < d->n_bucket)
+        return fail(VM_ERR_INVALID_ARG, "need 0 < n_bucket <= n_entry");
+    if (d->n_block <= 0 || d->n_block > (1 << 22))
+        return fail(VM_ERR_INVALID_ARG, "n_block must be in [1, 2^22] (ptr = block * 512 is an int)");
+    if (!(d->voxel_size > 0.f) || !(d->trunc_dist > 0.f) || !std::isfinite(d->voxel_size) ||
+        !std::isfinite(d->trunc_dist))
+        return fail(VM_ERR_INVALID_ARG, "voxel_size and trunc_dist must be finite and > 0");
+    if (!(d->depth_min >= 0.f) || !(d->depth_max > d->depth_min))
+        return fail(VM_ERR_INVALID_ARG, "need 0 <= depth_min < depth_max");
+    if (d->voxel_format < 0 || d->voxel_format > 2) return fail(VM_ERR_INVALID_ARG, "unknown voxel_format");
+    if (d->voxel_format == VM_VOXEL_F32_F32_RGB8 && !(d->max_weight >= 1.f))
+        return fail(VM_ERR_INVALID_ARG, "max_weight must be >= 1");
+
+    vm_map* m = new vm_map;
+    m->desc = *d;
+    if (m->desc.voxel_format != VM_VOXEL_F32_F32_RGB8) m->desc.max_weight = 255.f;
+    MapView& v = m->view;
+    std::memset(&v, 0, sizeof(v));
+    v.nEntry = d->n_entry;
+    v.nBucket = d->n_bucket;
+    v.nBlock = d->n_block;
+    v.bucketMask = (d->n_bucket & (d->n_bucket - 1)) == 0 ? d->n_bucket - 1 : -1;
+    v.voxelSize = d->voxel_size;
+    v.trunc = d->trunc_dist;
+    v.depthMin = d->depth_min;
+    v.depthMax = d->depth_max;
+    v.maxWeight = m->desc.max_weight;
+    v.fmt = d->voxel_format;
+    hipGetDevice(&m->device);
+
+    const size_t vb = (size_t)voxel_bytes(v.fmt);
+    struct { void** p; size_t bytes; const char* name; } allocs[] = {
+        {(void**)&v.table, sizeof(HashEntry) * (size_t)v.nEntry, "hash table"},
+        {(void**)&v.visible, sizeof(HashEntry) * (size_t)v.nEntry, "visible table"},
+        {(void**)&v.heap, sizeof(int) * (size_t)v.nBlock, "heap"},
+        {(void**)&v.mutex, sizeof(int) * (size_t)v.nBucket, "bucket mutex"},
+        {(void**)&v.voxels, vb * kBlockVox * (size_t)v.nBlock, "voxel blocks"},
+        {(void**)&v.blockPos, sizeof(int4) * (size_t)v.nBlock, "block positions"},
+        {(void**)&v.ctr, sizeof(int) * kCounterInts, "counters"},
+    };
+    for (auto& a : allocs) {
+        hipError_t e = hipMalloc(a.p, a.bytes);
+        if (e != hipSuccess) {
+            std::string msg = std::string("hipMalloc of ") + a.name + " failed: " + hipGetErrorString(e);
+            vm_map_destroy(m);
+            return fail(VM_ERR_OUT_OF_MEMORY, msg);
+        }
+    }
+    int rc = vm_map_reset(m, nullptr);
+    if (rc != VM_OK) {
+        std::string msg = g_error;
+        vm_map_destroy(m);
+        return fail(rc, msg);
+    }
+    hipError_t e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+        vm_map_destroy(m);
+        return hip_fail(e, "map reset");
+    }
+    *out = m;
+    return VM_OK;
+}
+
+int vm_map_destroy(vm_map* m) {
+    if (!m) return VM_OK;
+    MapView& v = m->view;
+    void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete m;
+    return VM_OK;
+}
+
+int vm_map_reset(vm_map* m, void* stream) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    VM_HIP(launch_reset(m->view, (hipStream_t)stream), "map reset");
+    m->epoch++;
+    m->visValid = false;
+    return VM_OK;
+}
+
+int vm_map_get_desc(const vm_map* m, vm_map_desc* out) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (!out) return fail(VM_ERR_INVALID_ARG, "out_desc is NULL");
+    *out = m->desc;
+    return VM_OK;
+}
+
+int vm_map_get_arrays(const vm_map* m, vm_map_arrays* o) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (!o) return fail(VM_ERR_INVALID_ARG, "out_arrays is NULL");
+    const MapView& v = m->view;
+    o->hash_table = v.table;
+    o->visible_table = v.visible;
+    o->heap = v.heap;
+    o->heap_ptr = v.ctr + kHeapPtr;
+    o->excess_ptr = v.ctr + kExcessPtr;
+    o->bucket_mutex = v.mutex;
+    o->visible_block_num = (uint32_t*)(v.ctr + kVisibleCount);
+    o->voxel_block = v.voxels;
+    o->block_pos = (int32_t*)v.blockPos;
+    return VM_OK;
+}
+
+int vm_map_get_stats(vm_map* m, void* stream, vm_map_stats* o) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (!o) return fail(VM_ERR_INVALID_ARG, "out_stats is NULL");
+    int c[kCounterInts];
+    hipStream_t s = (hipStream_t)stream;
+    VM_HIP(hipMemcpyAsync(c, m->view.ctr, sizeof(c), hipMemcpyDeviceToHost, s), "counter copy");
+    VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    o->heap_ptr = c[kHeapPtr];
+    o->allocated_blocks = m->view.nBlock - 1 - c[kHeapPtr];
+    o->excess_used = c[kExcessPtr];
+    o->visible_blocks = c[kVisibleCount];
+    o->status = c[kStatus];
+    o->frames_fused = c[kFramesFused];
+    uint64_t lo = (uint32_t)c[kSumVisibleLo], hi = (uint32_t)c[kSumVisibleHi];
+    o->sum_visible = (int64_t)(lo | (hi << 32));
+    return VM_OK;
+}
+
+int vm_allocate_blocks(vm_map* m, const void* depth, int64_t pitch, int32_t W, int32_t H, int32_t fmt, float scale,
+                       const float K[4], const float T[16], void* stream) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (fmt != VM_DEPTH_F32 && fmt != VM_DEPTH_U16) return fail(VM_ERR_INVALID_ARG, "unknown depth_format");
+    if (int rc = check_image(depth, pitch, W, H, fmt == VM_DEPTH_F32 ? 4 : 2, "depth")) return rc;
+    if (int rc = check_pose_k(K, T)) return rc;
+    DepthImage d{(const uint8_t*)depth, pitch, W, H, fmt, scale};
+    VM_HIP(launch_allocate(m->view, d, intrinsics_of(K), affine_of(T), (hipStream_t)stream), "allocation");
+    m->epoch++;
+    m->visCountZeroed = true;
+    return VM_OK;
+}
+
+int vm_integrate(vm_map* m, const void* depth, int64_t pitch, int32_t W, int32_t H, int32_t fmt, float scale,
+                 const void* rgb, int64_t rgb_pitch, const float K[4], const float T[16], void* stream,
+                 uint32_t* out_visible) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (fmt != VM_DEPTH_F32 && fmt != VM_DEPTH_U16) return fail(VM_ERR_INVALID_ARG, "unknown depth_format");
+    if (int rc = check_image(depth, pitch, W, H, fmt == VM_DEPTH_F32 ? 4 : 2, "depth")) return rc;
+    if (rgb)
+        if (int rc = check_image(rgb, rgb_pitch, W, H, 3, "rgb")) return rc;
+    if (int rc = check_pose_k(K, T)) return rc;
+    float Ti[16];
+    if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
+    hipStream_t s = (hipStream_t)stream;
+    // the allocation kernel zeroes the visible count; without it, zero it here
+    if (!m->visCountZeroed) VM_HIP(launch_fill_u32(m->view.ctr + kVisibleCount, 0u, 1, s), "count reset");
+    m->visCountZeroed = false;
+    DepthImage d{(const uint8_t*)depth, pitch, W, H, fmt, scale};
+    ColorImage c{(const uint8_t*)rgb, rgb_pitch};
+    VM_HIP(launch_visible_fuse(m->view, d, c, intrinsics_of(K), affine_of(Ti), s), "visibility + integration");
+    remember_visibility(m, W, H, K, T);
+    if (out_visible) {
+        int n = 0;
+        if (int rc = read_counter(m, kVisibleCount, s, &n)) return rc;
+        *out_visible = (uint32_t)n;
+    }
+    return VM_OK;
+}
+
+int vm_fuse_depth(vm_map* m, const void* depth, int64_t pitch, int32_t W, int32_t H, int32_t fmt, float scale,
+                  const void* rgb, int64_t rgb_pitch, const float K[4], const float T[16], void* stream,
+                  uint32_t* out_visible) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (fmt != VM_DEPTH_F32 && fmt != VM_DEPTH_U16) return fail(VM_ERR_INVALID_ARG, "unknown depth_format");
+    if (int rc = check_image(depth, pitch, W, H, fmt == VM_DEPTH_F32 ? 4 : 2, "depth")) return rc;
+    if (rgb)
+        if (int rc = check_image(rgb, rgb_pitch, W, H, 3, "rgb")) return rc;
+    if (int rc = check_pose_k(K, T)) return rc;
+    float Ti[16];
+    if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
+    hipStream_t s = (hipStream_t)stream;
+    const Intrinsics k = intrinsics_of(K);
+    DepthImage d{(const uint8_t*)depth, pitch, W, H, fmt, scale};
+    ColorImage c{(const uint8_t*)rgb, rgb_pitch};
+    VM_HIP(launch_allocate(m->view, d, k, affine_of(T), s), "allocation");
+    m->epoch++;
+    VM_HIP(launch_visible_fuse(m->view, d, c, k, affine_of(Ti), s), "visibility + integration");
+    remember_visibility(m, W, H, K, T);
+    if (out_visible) {
+        int n = 0;
+        if (int rc = read_counter(m, kVisibleCount, s, &n)) return rc;
+        *out_visible = (uint32_t)n;
+    }
+    return VM_OK;
+}
+
+int vm_check_visible_blocks(vm_map* m, int32_t W, int32_t H, const float K[4], const float T[16], void* stream,
+                            uint32_t* out_visible) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (W <= 0 || H <= 0) return fail(VM_ERR_INVALID_ARG, "width and height must be > 0");
+    if (int rc = check_pose_k(K, T)) return rc;
+    float Ti[16];
+    if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
+    hipStream_t s = (hipStream_t)stream;
+    VM_HIP(launch_visible(m->view, W, H, intrinsics_of(K), affine_of(Ti), s), "visibility");
+    remember_visibility(m, W, H, K, T);
+    if (out_visible) {
+        int n = 0;
+        if (int rc = read_counter(m, kVisibleCount, s, &n)) return rc;
+        *out_visible = (uint32_t)n;
+    }
+    return VM_OK;
+}
+
+int vm_raycast(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H, const float K[4], const float T[16],
+               void* stream) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (int rc = check_image(vmap, pitch, W, H, 16, "vmap")) return rc;
+    if (pitch % 16) return fail(VM_ERR_INVALID_ARG, "vmap pitch must be a multiple of 16");
+    if (int rc = check_pose_k(K, T)) return rc;
+    float Ti[16];
+    if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
+    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, intrinsics_of(K), affine_of(T), affine_of(Ti), false,
+                          (hipStream_t)stream),
+           "raycast");
+    return VM_OK;
+}
+
+int vm_render_scene(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H, const float K[4],
+                    const float T[16], void* stream) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (int rc = check_image(vmap, pitch, W, H, 16, "vmap")) return rc;
+    if (pitch % 16) return fail(VM_ERR_INVALID_ARG, "vmap pitch must be a multiple of 16");
+    if (int rc = check_pose_k(K, T)) return rc;
+    float Ti[16];
+    if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
+    hipStream_t s = (hipStream_t)stream;
+    const Intrinsics k = intrinsics_of(K);
+    if (!visibility_current(m, W, H, K, T)) {
+        VM_HIP(launch_visible(m->view, W, H, k, affine_of(Ti), s), "visibility");
+        remember_visibility(m, W, H, K, T);
+    }
+    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s), "raycast");
+    return VM_OK;
+}
+
+int vm_project_rendering_blocks(vm_map* m, float* zx, float* zy, int64_t pitch, int32_t cols8, int32_t rows8,
+                                const float K[4], const float Twc[16], void* stream, uint32_t* out_count) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (int rc = check_image(zx, pitch, cols8, rows8, 4, "zrange_x")) return rc;
+    if (int rc = check_image(zy, pitch, cols8, rows8, 4, "zrange_y")) return rc;
+    if (int rc = check_pose_k(K, Twc)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    VM_HIP(launch_rendering_blocks(m->view, zx, zy, pitch, cols8, rows8, intrinsics_of(K), affine_of(Twc), s),
+           "rendering blocks");
+    if (out_count) {
+        int n = 0;
+        if (int rc = read_counter(m, kRenderBlockCount, s, &n)) return rc;
+        *out_count = (uint32_t)n;
+    }
+    return VM_OK;
+}
+
+int vm_compute_normals(const float* vmap, int64_t vp, float* nmap, int64_t np, int32_t W, int32_t H,
+                       void* stream) {
+    if (int rc = check_image(vmap, vp, W, H, 16, "vmap")) return rc;
+    if (int rc = check_image(nmap, np, W, H, 16, "nmap")) return rc;
+    if (vp % 16 || np % 16) return fail(VM_ERR_INVALID_ARG, "vmap/nmap pitch must be a multiple of 16");
+    VM_HIP(launch_normals(vmap, vp, nmap, np, W, H, (hipStream_t)stream), "normals");
+    return VM_OK;
+}
+
+int vm_render_shading(const float* vmap, int64_t vp, const float* nmap, int64_t np, uint8_t* rgba, int64_t rp,
+                      int32_t W, int32_t H, void* stream) {
+    if (int rc = check_image(vmap, vp, W, H, 16, "vmap")) return rc;
+    if (int rc = check_image(nmap, np, W, H, 16, "nmap")) return rc;
+    if (int rc = check_image(rgba, rp, W, H, 4, "image")) return rc;
+    if (vp % 16 || np % 16 || rp % 4) return fail(VM_ERR_INVALID_ARG, "misaligned pitch");
+    VM_HIP(launch_shading(vmap, vp, nmap, np, rgba, rp, W, H, (hipStream_t)stream), "shading");
+    return VM_OK;
+}
+
+int vm_polygonize(vm_map* m, float* verts, float* norms, int64_t cap, void* stream, uint32_t* out_blocks,
+                  uint32_t* out_tris) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (cap < 0) return fail(VM_ERR_INVALID_ARG, "cap_triangles must be >= 0");
+    if (cap > 0 && (!verts || !norms)) return fail(VM_ERR_INVALID_ARG, "verts and norms must not be NULL");
+    hipStream_t s = (hipStream_t)stream;
+    VM_HIP(launch_polygonize(m->view, verts, norms, cap, s), "polygonize");
+    int c[kCounterInts];
+    VM_HIP(hipMemcpyAsync(c, m->view.ctr, sizeof(c), hipMemcpyDeviceToHost, s), "counter copy");
+    VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    const int64_t tris = (int64_t)(uint32_t)c[kTriangleCount];
+    if (tris > cap) {
+        int st = c[kStatus] | kStatusMeshClamped;
+        VM_HIP(hipMemcpy(m->view.ctr + kStatus, &st, sizeof(int), hipMemcpyHostToDevice), "status update");
+    }
+    if (out_blocks) *out_blocks = (uint32_t)(m->view.nBlock - 1 - c[kHeapPtr]);
+    if (out_tris) *out_tris = (uint32_t)(tris < cap ? tris : cap);
+    return VM_OK;
+}
+
+int vm_surface_points(const float* verts, float* points, int64_t n, void* stream) {
+    if (n < 0) return fail(VM_ERR_INVALID_ARG, "n_triangles must be >= 0");
+    if (n > 0 && (!verts || !points)) return fail(VM_ERR_INVALID_ARG, "verts and points must not be NULL");
+    VM_HIP(launch_surface_points(verts, points, n, (hipStream_t)stream), "surface points");
+    return VM_OK;
+}
+
+static int array_of(vm_map* m, int which, void** p, int64_t* bytes) {
+    const MapView& v = m->view;
+    switch (which) {
+        case 0: *p = v.table; *bytes = (int64_t)sizeof(HashEntry) * v.nEntry; return VM_OK;
+        case 1: *p = v.heap; *bytes = (int64_t)sizeof(int) * v.nBlock; return VM_OK;
+        case 2: *p = v.voxels; *bytes = (int64_t)voxel_bytes(v.fmt) * kBlockVox * v.nBlock; return VM_OK;
+        case 3: *p = v.ctr; *bytes = (int64_t)sizeof(int) * kCounterInts; return VM_OK;
+        case 4: *p = v.visible; *bytes = (int64_t)sizeof(HashEntry) * v.nEntry; return VM_OK;
+        case 5: *p = v.blockPos; *bytes = (int64_t)sizeof(int4) * v.nBlock; return VM_OK;
+        default: return fail(VM_ERR_INVALID_ARG, "unknown array id");
+    }
+}
+
+int vm_map_download(vm_map* m, int32_t which, void* dst, int64_t bytes) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    void* p;
+    int64_t n;
+    if (int rc = array_of(m, which, &p, &n)) return rc;
+    if (!dst || bytes != n) return fail(VM_ERR_INVALID_ARG, "host buffer size does not match the array");
+    VM_HIP(hipDeviceSynchronize(), "synchronize");
+    VM_HIP(hipMemcpy(dst, p, (size_t)n, hipMemcpyDeviceToHost), "download");
+    return VM_OK;
+}
+
+int vm_map_upload(vm_map* m, int32_t which, const void* src, int64_t bytes) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    void* p;
+    int64_t n;
+    if (int rc = array_of(m, which, &p, &n)) return rc;
+    if (!src || bytes != n) return fail(VM_ERR_INVALID_ARG, "host buffer size does not match the array");
+    VM_HIP(hipDeviceSynchronize(), "synchronize");
+    VM_HIP(hipMemcpy(p, src, (size_t)n, hipMemcpyHostToDevice), "upload");
+    if (which == 0) {
+        VM_HIP(launch_fill_u32(m->view.blockPos, (uint32_t)kEmptyKey, (int64_t)4 * m->view.nBlock, nullptr),
+               "block index clear");
+        VM_HIP(launch_rebuild_index(m->view, nullptr), "block index rebuild");
+        VM_HIP(hipStreamSynchronize(nullptr), "synchronize");
+    }
+    m->epoch++;
+    m->visValid = false;
+    return VM_OK;
+}
+
+}  // extern "C"

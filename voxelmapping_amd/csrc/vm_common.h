@@ -1,0 +1,163 @@
+// Shared device-side definitions for the gfx950 fusion pipeline.
+//
+// Arithmetic contract: every floating-point expression keeps the operand
+// order of the reference (include/map_struct.h, src/*.cu) and of Eigen >= 3.3
+// fixed-size reductions (a0 + (a1 + a2)); the library is compiled with
+// -ffp-contract=off and IEEE-rounded division/sqrt, so each expression rounds
+// exactly as written. Integer hashing and block/case indexing are exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <climits>
+
+namespace vmamd {
+
+constexpr int kBlockSide = 8;    // BLOCK_SIZE, map_struct.h:6
+constexpr int kBlockVox = 512;   // BLOCK_SIZE_3, map_struct.h:7
+constexpr int kEmptyKey = INT_MIN;  // position of a never-used hash slot
+constexpr int kClaimed = -2;        // ptr of a slot claimed by an insert in flight
+
+// HashEntry, map_struct.h:30-35: {int ptr; int offset; Vector3i pos} (20 B)
+struct HashEntry {
+    int ptr;
+    int offset;
+    int x, y, z;
+};
+static_assert(sizeof(HashEntry) == 20, "HashEntry layout");
+
+// Device counter block, one allocation per map. The reference keeps heapPtr,
+// excessPtr and visibleBlockNum as separate one-int allocations
+// (map_struct.cu:44-52); they live here at fixed indices.
+enum Counter : int {
+    kHeapPtr = 0,      // top of the free-block stack (map_struct.h:61)
+    kExcessPtr = 1,    // next excess-list offset (map_struct.h:60)
+    kVisibleCount = 2, // compacted visible entries (map_struct.h:67)
+    kStatus = 3,       // sticky status bits, see StatusBits
+    kTriangleCount = 4,
+    kRenderBlockCount = 5,
+    kFramesFused = 6,
+    kSumVisibleLo = 8, // 64-bit running sum of visible counts (stats)
+    kSumVisibleHi = 9,
+    kCounterInts = 16,
+};
+
+enum StatusBits : int {
+    kStatusHeapFull = 1,     // a block allocation found the heap empty
+    kStatusExcessFull = 2,   // an excess-list allocation found the list full
+    kStatusChainBound = 4,   // a chain walk exceeded the table length
+    kStatusMeshClamped = 8,  // polygonize produced more triangles than fit
+};
+
+enum VoxelFormat : int {
+    kFmtRef = 0,     // vmap::Voxel    {short sdf; uchar wt}           4 B
+    kFmtRefRGB = 1,  // vmap::VoxelRGB {short sdf; uchar wt; uchar3}   6 B
+    kFmtF32 = 2,     // {float sdf; float wt; uchar rgb[3]; uchar pad} 12 B
+};
+
+// Plain-old-data view of one map, passed to kernels by value.
+struct MapView {
+    HashEntry* table;      // nEntry slots: nBucket heads, then the excess list
+    HashEntry* visible;    // compacted copies of visible entries
+    int* heap;             // nBlock free-block stack
+    int* mutex;            // nBucket bucket locks
+    uint8_t* voxels;       // nBlock * 512 voxels of the map's format
+    int4* blockPos;        // block id -> {x, y, z, hash slot}; x = kEmptyKey if dead
+    int* ctr;              // Counter block
+    int nEntry, nBucket, nBlock;
+    int bucketMask;        // nBucket - 1 when nBucket is a power of two, else -1
+    float voxelSize, trunc, depthMin, depthMax, maxWeight;
+    int fmt;
+};
+
+struct Intrinsics {
+    float fx, fy, cx, cy;  // K(0,0), K(1,1), K(0,2), K(1,2)
+    float ifx, ify;        // (float)(1.0 / K(0,0)): double divide, depth_fusion.cu:209
+};
+
+// Affine part of a column-major 4x4: rows of R and t.
+struct Affine {
+    float r00, r01, r02, t0;
+    float r10, r11, r12, t1;
+    float r20, r21, r22, t2;
+};
+
+// ---------------------------------------------------------------------------
+// exact helpers (host + device)
+
+__host__ __device__ inline float sum3(float a, float b, float c) { return a + (b + c); }
+
+__host__ __device__ inline float3 xform(const Affine& T, float3 v) {
+    float3 o;
+    o.x = T.t0 + sum3(T.r00 * v.x, T.r01 * v.y, T.r02 * v.z);
+    o.y = T.t1 + sum3(T.r10 * v.x, T.r11 * v.y, T.r12 * v.z);
+    o.z = T.t2 + sum3(T.r20 * v.x, T.r21 * v.y, T.r22 * v.z);
+    return o;
+}
+
+// float -> int conversion with the hardware's semantics made explicit:
+// NaN -> 0, out-of-range saturates (v_cvt_i32_f32 behaviour).
+__host__ __device__ inline int f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT_MAX;
+    if (x < -2147483648.0f) return INT_MIN;
+    return (int)x;
+}
+__host__ __device__ inline int floor_i(float x) { return f2i_sat(floorf(x)); }
+
+// hash, map_struct.h:104-109: wrap-around int32 products, C remainder made
+// non-negative. For a power-of-two bucket count the remainder is a mask.
+__host__ __device__ inline int hash_block(int x, int y, int z, int nBucket, int mask) {
+    uint32_t r = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349669u) ^ ((uint32_t)z * 83492791u);
+    if (mask >= 0) return (int)(r & (uint32_t)mask);
+    int res = (int)r % nBucket;
+    return res < 0 ? res + nBucket : res;
+}
+
+// voxelPosToBlockPos (map_struct.h:131-141) is floor division by 8, i.e. an
+// arithmetic shift; voxelPosToLocalPos (:148-162) is the Euclidean remainder.
+__host__ __device__ inline int vox_to_blk(int v) { return v >> 3; }
+__host__ __device__ inline int vox_to_loc(int v) { return v & 7; }
+// localPosToLocalIdx, map_struct.h:164-167
+__host__ __device__ inline int local_index(int lx, int ly, int lz) { return (lz << 6) | (ly << 3) | lx; }
+
+__host__ __device__ inline int voxel_bytes(int fmt) { return fmt == kFmtRef ? 4 : fmt == kFmtRefRGB ? 6 : 12; }
+
+// ---------------------------------------------------------------------------
+// voxel storage traits
+
+template <int FMT> struct Vox;
+
+// vmap::Voxel: bytes 0-1 sdf (int16, TSDF * 32767), byte 2 weight, byte 3 pad
+template <> struct Vox<kFmtRef> {
+    static constexpr int kBytes = 4;
+    static constexpr bool kHasColor = false;
+    __device__ static inline float sdf_of(int16_t s) { return (float)s / 32767.0f; }
+    __device__ static inline void read(const uint8_t* base, int64_t i, float& sdf, bool& valid) {
+        uint32_t w = *(const uint32_t*)(base + i * 4);
+        sdf = sdf_of((int16_t)(w & 0xffffu));
+        valid = ((w >> 16) & 0xffu) != 0;
+    }
+};
+// vmap::VoxelRGB: bytes 0-1 sdf, 2 weight, 3-5 rgb (6 B, 2-byte aligned)
+template <> struct Vox<kFmtRefRGB> {
+    static constexpr int kBytes = 6;
+    static constexpr bool kHasColor = true;
+    __device__ static inline void read(const uint8_t* base, int64_t i, float& sdf, bool& valid) {
+        const uint16_t* p = (const uint16_t*)(base + i * 6);
+        sdf = (float)(int16_t)p[0] / 32767.0f;
+        valid = (p[1] & 0xffu) != 0;
+    }
+};
+// float TSDF + float weight + rgb8
+template <> struct Vox<kFmtF32> {
+    static constexpr int kBytes = 12;
+    static constexpr bool kHasColor = true;
+    __device__ static inline void read(const uint8_t* base, int64_t i, float& sdf, bool& valid) {
+        const float* p = (const float*)(base + i * 12);
+        sdf = p[0];
+        valid = p[1] != 0.f;
+    }
+};
+
+}  // namespace vmamd

@@ -1,0 +1,62 @@
+// Host-side launchers of the gfx950 kernels (one translation unit per stage).
+#pragma once
+
+#include "vm_common.h"
+
+namespace vmamd {
+
+enum DepthFormat : int { kDepthF32 = 0, kDepthU16 = 1 };
+
+struct DepthImage {
+    const uint8_t* data;
+    int64_t pitch;  // bytes between rows
+    int width, height;
+    int format;     // DepthFormat
+    float scale;    // metres per unit for kDepthU16
+};
+
+struct ColorImage {
+    const uint8_t* data;  // 3 bytes per pixel, or null
+    int64_t pitch;
+};
+
+// map_struct.cu:29-42 equivalent (plus the block-position index)
+hipError_t launch_reset(const MapView& m, hipStream_t s);
+// rebuild blockPos from the hash table (after state is loaded from outside)
+hipError_t launch_rebuild_index(const MapView& m, hipStream_t s);
+// CreateBlockLineTracingFunctor, depth_fusion.cu:8-60; also zeroes the visible count
+hipError_t launch_allocate(const MapView& m, const DepthImage& d, const Intrinsics& K,
+                           const Affine& camToWorld, hipStream_t s);
+// CheckEntryVisibilityFunctor + DepthFusionFunctor/ImageFusionFunctor fused
+hipError_t launch_visible_fuse(const MapView& m, const DepthImage& d, const ColorImage& c,
+                               const Intrinsics& K, const Affine& worldToCam, hipStream_t s);
+// CheckEntryVisibilityFunctor alone (CheckVisibleBlocks, depth_fusion.cu:494-539)
+hipError_t launch_visible(const MapView& m, int width, int height, const Intrinsics& K,
+                          const Affine& worldToCam, hipStream_t s);
+// MapRenderingDelegate, ray_tracing.cu:255-401. gate: skip when the visible
+// count is zero (the facade's gate, vmapping.cpp:87-88 / :101)
+hipError_t launch_raycast(const MapView& m, float* vmap, int64_t pitch, int width, int height,
+                          const Intrinsics& K, const Affine& camToWorld, const Affine& worldToCam,
+                          bool gate, hipStream_t s);
+// RenderingBlockDelegate + FillRenderingBlockFunctor, ray_tracing.cu:13-234
+hipError_t launch_rendering_blocks(const MapView& m, float* zx, float* zy, int64_t pitch,
+                                   int cols8, int rows8, const Intrinsics& K,
+                                   const Affine& worldToCam, hipStream_t s);
+// computeNormalKernel, image_proc.cu:74-104
+hipError_t launch_normals(const float* vmap, int64_t vpitch, float* nmap, int64_t npitch,
+                          int width, int height, hipStream_t s);
+// renderSceneKernel, image_proc.cu:46-72
+hipError_t launch_shading(const float* vmap, int64_t vpitch, const float* nmap, int64_t npitch,
+                          uint8_t* rgba, int64_t rpitch, int width, int height, hipStream_t s);
+// BuildVertexArray / Polygonize, marching_cubes.cu:9-305
+hipError_t launch_polygonize(const MapView& m, float* verts, float* norms, int64_t capTriangles,
+                             hipStream_t s);
+// compute_surface_point_kernel, marching_cubes.cu:314-344
+hipError_t launch_surface_points(const float* verts, float* points, int64_t n, hipStream_t s);
+
+// device-side fill of 32-bit words (graph-capturable replacement of memset)
+hipError_t launch_fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t s);
+
+int num_compute_units();
+
+}  // namespace vmamd

@@ -1,0 +1,329 @@
+"""Python mirror of the reference's fusion API on top of the C ABI.
+
+* :class:`DeviceMap` mirrors ``MapStruct<TVoxel>`` (include/map_struct.h:44-68)
+  with the stage functions of src/device_functions.h:11-66 as methods.
+* :class:`VoxelMapping` mirrors ``vmap::VoxelMapping`` (include/vmapping.h:10-27,
+  src/vmapping.cpp:150-190), same method names.
+* :func:`computeNormal` / :func:`renderScene` mirror include/image_proc.h:10-11.
+
+Images are torch CUDA tensors (device memory); poses are 4x4 camera-to-world
+matrices (numpy or torch, row-major as printed) or column-major float[16]
+arrays. Every call goes to the gfx950 kernels; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+def _stream(stream):
+    if stream is None:
+        if torch is not None and torch.cuda.is_available():
+            return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+def pose_colmajor(T) -> "C.Array":
+    """4x4 matrix (row-major indexing) or 16 column-major floats -> float[16]."""
+    a = np.asarray(T.cpu() if hasattr(T, "cpu") else T, dtype=np.float32)
+    flat = a.T.reshape(-1) if a.shape == (4, 4) else a.reshape(-1)
+    if flat.size != 16:
+        raise ValueError("pose must be 4x4 or 16 column-major floats")
+    return _capi.floats(flat, 16)
+
+
+def intrinsics(K) -> "C.Array":
+    """3x3 K or (fx, fy, cx, cy) -> float[4]."""
+    a = np.asarray(K.cpu() if hasattr(K, "cpu") else K, dtype=np.float32)
+    if a.shape == (3, 3):
+        a = np.array([a[0, 0], a[1, 1], a[0, 2], a[1, 2]], np.float32)
+    if a.size != 4:
+        raise ValueError("K must be 3x3 or (fx, fy, cx, cy)")
+    return _capi.floats(a.reshape(-1), 4)
+
+
+def _image(t, channels_bytes: int, name: str):
+    """(ptr, pitch, width, height) of a CUDA tensor laid out HxW[xC] with unit inner stride."""
+    if torch is None or not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor in device memory")
+    if not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA (HIP) tensor")
+    if t.dim() < 2:
+        raise ValueError(f"{name} must be at least 2-D")
+    H, W = t.shape[0], t.shape[1]
+    es = t.element_size()
+    row_elems = 1
+    for d in range(t.dim() - 1, 0, -1):
+        if t.stride(d) != row_elems:
+            raise ValueError(f"{name} rows must be contiguous")
+        row_elems *= t.shape[d]
+    if row_elems * es != W * channels_bytes:
+        raise ValueError(f"{name}: expected {channels_bytes} bytes per pixel")
+    return C.c_void_p(t.data_ptr()), t.stride(0) * es, W, H
+
+
+class DeviceMap:
+    """MapStruct<TVoxel>: hash table + excess list + block heap + voxels on the GPU."""
+
+    def __init__(self, n_entry: int, n_bucket: int, n_block: int, voxel_size: float, trunc_dist: float,
+                 voxel_format: int = _capi.VOXEL_I16_U8, depth_min: float = 0.1, depth_max: float = 5.0,
+                 max_weight: float = 255.0):
+        d = _capi.MapDesc(n_entry, n_bucket, n_block, voxel_size, trunc_dist, depth_min, depth_max,
+                          voxel_format, max_weight)
+        h = C.c_void_p()
+        check(lib().vm_map_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.desc = d
+
+    # MapStruct::create / release / reset / empty
+    @classmethod
+    def create(cls, nEntry, nBucket, nVBlock, voxelSize, truncationDist, **kw):
+        return cls(nEntry, nBucket, nVBlock, voxelSize, truncationDist, **kw)
+
+    def release(self):
+        if getattr(self, "_h", None):
+            lib().vm_map_destroy(self._h)
+            self._h = None
+
+    __del__ = release
+
+    def empty(self) -> bool:
+        return not self._h
+
+    def reset(self, stream=None):
+        check(lib().vm_map_reset(self._h, _stream(stream)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def voxel_bytes(self) -> int:
+        return {0: 4, 1: 6, 2: 12}[self.desc.voxel_format]
+
+    def stats(self, stream=None) -> dict:
+        s = _capi.MapStats()
+        check(lib().vm_map_get_stats(self._h, _stream(stream), C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _capi.MapStats._fields_}
+
+    def getVisibleBlockCount(self) -> int:
+        return self.stats()["visible_blocks"]
+
+    def arrays(self) -> dict:
+        a = _capi.MapArrays()
+        check(lib().vm_map_get_arrays(self._h, C.byref(a)))
+        return {k: getattr(a, k) for k, _ in _capi.MapArrays._fields_}
+
+    # --- stages (device_functions.h) ---------------------------------------
+    def _depth(self, depth, depth_scale):
+        if depth.dtype == torch.float32:
+            p, pitch, W, H = _image(depth, 4, "depth")
+            return p, pitch, W, H, _capi.DEPTH_F32, 1.0
+        if depth.dtype in (torch.uint16, torch.int16):
+            p, pitch, W, H = _image(depth, 2, "depth")
+            return p, pitch, W, H, _capi.DEPTH_U16, float(depth_scale)
+        raise TypeError("depth must be float32 metres or uint16 raw units")
+
+    def fuse(self, depth, K, cam_to_world, rgb=None, depth_scale: float = 1e-3, stream=None,
+             return_count: bool = False):
+        """FuseImage / FuseDepthAndImage. Returns the visible-block count if asked (synchronises)."""
+        p, pitch, W, H, fmt, sc = self._depth(depth, depth_scale)
+        rp, rpitch = None, 0
+        if rgb is not None:
+            rp, rpitch, rw, rh = _image(rgb, 3, "rgb")
+            if (rw, rh) != (W, H):
+                raise ValueError("rgb and depth sizes differ")
+        cnt = C.c_uint32(0)
+        check(lib().vm_fuse_depth(self._h, p, pitch, W, H, fmt, sc, rp, rpitch, intrinsics(K),
+                                  pose_colmajor(cam_to_world), _stream(stream),
+                                  C.byref(cnt) if return_count else None))
+        return int(cnt.value) if return_count else None
+
+    def allocate(self, depth, K, cam_to_world, depth_scale: float = 1e-3, stream=None):
+        p, pitch, W, H, fmt, sc = self._depth(depth, depth_scale)
+        check(lib().vm_allocate_blocks(self._h, p, pitch, W, H, fmt, sc, intrinsics(K),
+                                       pose_colmajor(cam_to_world), _stream(stream)))
+
+    def integrate(self, depth, K, cam_to_world, rgb=None, depth_scale: float = 1e-3, stream=None,
+                  return_count: bool = False):
+        """Visibility + integration after :meth:`allocate` (the rest of FuseImage)."""
+        p, pitch, W, H, fmt, sc = self._depth(depth, depth_scale)
+        rp, rpitch = None, 0
+        if rgb is not None:
+            rp, rpitch, rw, rh = _image(rgb, 3, "rgb")
+            if (rw, rh) != (W, H):
+                raise ValueError("rgb and depth sizes differ")
+        cnt = C.c_uint32(0)
+        check(lib().vm_integrate(self._h, p, pitch, W, H, fmt, sc, rp, rpitch, intrinsics(K),
+                                 pose_colmajor(cam_to_world), _stream(stream),
+                                 C.byref(cnt) if return_count else None))
+        return int(cnt.value) if return_count else None
+
+    def check_visible(self, width, height, K, cam_to_world, stream=None) -> int:
+        cnt = C.c_uint32(0)
+        check(lib().vm_check_visible_blocks(self._h, width, height, intrinsics(K), pose_colmajor(cam_to_world),
+                                            _stream(stream), C.byref(cnt)))
+        return int(cnt.value)
+
+    def raycast(self, vmap, K, cam_to_world, stream=None):
+        p, pitch, W, H = _image(vmap, 16, "vmap")
+        check(lib().vm_raycast(self._h, p, pitch, W, H, intrinsics(K), pose_colmajor(cam_to_world),
+                               _stream(stream)))
+        return vmap
+
+    def render_scene(self, vmap, K, cam_to_world, stream=None):
+        p, pitch, W, H = _image(vmap, 16, "vmap")
+        check(lib().vm_render_scene(self._h, p, pitch, W, H, intrinsics(K), pose_colmajor(cam_to_world),
+                                    _stream(stream)))
+        return vmap
+
+    def project_rendering_blocks(self, zrange_x, zrange_y, K, world_to_cam, stream=None) -> int:
+        px, pitch, W8, H8 = _image(zrange_x, 4, "zrange_x")
+        py, pitch2, _, _ = _image(zrange_y, 4, "zrange_y")
+        if pitch2 != pitch:
+            raise ValueError("zrange maps must share a pitch")
+        cnt = C.c_uint32(0)
+        check(lib().vm_project_rendering_blocks(self._h, px, py, pitch, W8, H8, intrinsics(K),
+                                                pose_colmajor(world_to_cam), _stream(stream), C.byref(cnt)))
+        return int(cnt.value)
+
+    def polygonize(self, verts, norms, stream=None):
+        """Marching cubes into device buffers of shape (cap, 9). Returns (blocks, triangles)."""
+        cap = verts.shape[0]
+        if norms.shape[0] != cap or verts.numel() != cap * 9 or norms.numel() != cap * 9:
+            raise ValueError("verts and norms must be (cap, 9) float32")
+        b, t = C.c_uint32(0), C.c_uint32(0)
+        check(lib().vm_polygonize(self._h, C.c_void_p(verts.data_ptr()), C.c_void_p(norms.data_ptr()), cap,
+                                  _stream(stream), C.byref(b), C.byref(t)))
+        return int(b.value), int(t.value)
+
+    # --- state transfer -------------------------------------------------------
+    def _nbytes(self, which):
+        d = self.desc
+        return {0: 20 * d.n_entry, 1: 4 * d.n_block, 2: self.voxel_bytes * 512 * d.n_block, 3: 64,
+                4: 20 * d.n_entry, 5: 16 * d.n_block}[which]
+
+    def download(self, which: int) -> np.ndarray:
+        n = self._nbytes(which)
+        out = np.empty(n, np.uint8)
+        check(lib().vm_map_download(self._h, which, out.ctypes.data_as(C.c_void_p), n))
+        return out
+
+    def upload(self, which: int, data: np.ndarray):
+        data = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+        check(lib().vm_map_upload(self._h, which, data.ctypes.data_as(C.c_void_p), data.size))
+
+    def hash_table(self) -> np.ndarray:
+        return self.download(0).view(np.int32).reshape(-1, 5)
+
+    def counters(self) -> np.ndarray:
+        return self.download(3).view(np.int32)
+
+
+def computeNormal(vmap, nmap=None, stream=None):
+    """vmap::computeNormal (image_proc.cu:95-104)."""
+    if nmap is None:
+        nmap = torch.empty_like(vmap)
+    p, pitch, W, H = _image(vmap, 16, "vmap")
+    q, qpitch, W2, H2 = _image(nmap, 16, "nmap")
+    if (W, H) != (W2, H2):
+        raise ValueError("vmap and nmap sizes differ")
+    check(lib().vm_compute_normals(p, pitch, q, qpitch, W, H, _stream(stream)))
+    return nmap
+
+
+def renderScene(vmap, nmap, image=None, stream=None):
+    """vmap::renderScene (image_proc.cu:63-72): Phong shading to RGBA8."""
+    H, W = vmap.shape[0], vmap.shape[1]
+    if image is None:
+        image = torch.empty((H, W, 4), dtype=torch.uint8, device=vmap.device)
+    p, pitch, _, _ = _image(vmap, 16, "vmap")
+    q, qpitch, _, _ = _image(nmap, 16, "nmap")
+    r, rpitch, _, _ = _image(image, 4, "image")
+    check(lib().vm_render_shading(p, pitch, q, qpitch, r, rpitch, W, H, _stream(stream)))
+    return image
+
+
+def surface_points(verts, n_triangles: int, points=None, stream=None):
+    """GetSurfacePoints (marching_cubes.cu:330-344): triangle centroids."""
+    if points is None:
+        points = torch.empty((max(n_triangles, 1), 3), dtype=torch.float32, device=verts.device)
+    check(lib().vm_surface_points(C.c_void_p(verts.data_ptr()), C.c_void_p(points.data_ptr()), n_triangles,
+                                  _stream(stream)))
+    return points[:n_triangles]
+
+
+MAX_VERTS_BUFFER = 20_000_000  # triangles, vmapping.cpp:6
+
+
+class VoxelMapping:
+    """vmap::VoxelMapping (include/vmapping.h:10-27). Same methods and defaults:
+    CreateMap uses nBucket = int(0.9 * numVoxels), truncation = 4 * voxelSize and
+    the reference Voxel format (vmapping.cpp:57-75)."""
+
+    def __init__(self, w: int, h: int, K, mesh_capacity: int = MAX_VERTS_BUFFER):
+        self.width, self.height = int(w), int(h)
+        self.K = intrinsics(K)
+        self.map: DeviceMap | None = None
+        self.numVisBlock = 0
+        self.mesh_capacity = int(mesh_capacity)
+        self._verts = self._norms = None
+
+    def reset(self):
+        self._need_map().reset()
+
+    def CreateMap(self, numEntries: int, numVoxels: int, voxelSize: float = 0.01):
+        nb = int(numVoxels * 0.9)
+        self.map = DeviceMap(numEntries, nb, numVoxels, voxelSize, voxelSize * 4)
+
+    def _need_map(self) -> DeviceMap:
+        if self.map is None:
+            raise RuntimeError("CreateMap must be called first")
+        return self.map
+
+    def FuseDepth(self, depth, camToWorld):
+        self.numVisBlock = self._need_map().fuse(depth, self.K, camToWorld, return_count=True)
+
+    def FuseDepthAndImage(self, rgb, depth, camToWorld):
+        """The reference leaves this a stub (vmapping.cpp:167-170); here it fuses colour
+        when the map's format carries colour (FuseDepthAndImage, depth_fusion.cu:383-488)."""
+        self.numVisBlock = self._need_map().fuse(depth, self.K, camToWorld, rgb=rgb, return_count=True)
+
+    def RenderScene(self, vmap, camToWorld):
+        m = self._need_map()
+        if vmap is None:
+            vmap = torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
+        m.render_scene(vmap, self.K, camToWorld)
+        return vmap
+
+    def _mesh(self):
+        m = self._need_map()
+        if self._verts is None:
+            self._verts = torch.empty((self.mesh_capacity, 9), dtype=torch.float32, device="cuda")
+            self._norms = torch.empty((self.mesh_capacity, 9), dtype=torch.float32, device="cuda")
+        blocks, tris = m.polygonize(self._verts, self._norms)
+        self.numVisBlock = blocks
+        return tris
+
+    def Polygonize(self):
+        """Returns (triangle_count, verts[T*9], norms[T*9]) as host arrays."""
+        t = self._mesh()
+        return t, self._verts[:t].cpu().numpy().reshape(-1), self._norms[:t].cpu().numpy().reshape(-1)
+
+    def GetSurfacePoints(self):
+        t = self._mesh()
+        if t == 0:
+            return np.zeros((0, 3), np.float32)
+        return surface_points(self._verts, t).cpu().numpy()

@@ -1,0 +1,144 @@
+"""Seeded synthetic RGB-D scenes (ctypes binding of libvm_synth.so).
+
+The workload definitions used by the tests and the benchmark live here:
+
+* :func:`room_small` -- 4x4x3 m room, 5 spheres, 3 boxes, 320x240 float
+  metres depth (fx = fy = 262.5), 50 frames on a smooth circular trajectory,
+  1 cm voxels, 3 cm truncation, 2^17 buckets, 2^16-block heap.
+* :func:`room_vga` -- the same generator at 640x480 (fx = fy = 525), depth
+  as uint16 millimetres with scale 1/1000 and a 4 m cut, 500 frames of
+  seeded random walk (<= 2 cm, <= 1 degree per frame), 5 mm voxels, 2 cm
+  truncation, 2^20 buckets + 2^17 excess slots, 2^20-block heap, float TSDF,
+  float weight and uint8 colour.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SYNTH_PATH = os.path.join(_HERE, "libvm_synth.so")
+_lib = None
+
+
+def _synth_lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError(f"{SYNTH_PATH} is missing: run `python -m voxelmapping_amd.build`")
+        L = C.CDLL(SYNTH_PATH)
+        L.vm_synth_create.restype = C.c_void_p
+        L.vm_synth_create.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.vm_synth_destroy.argtypes = [C.c_void_p]
+        L.vm_synth_trajectory.restype = C.c_int
+        L.vm_synth_trajectory.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_double,
+                                          C.c_void_p]
+        L.vm_synth_render.restype = C.c_int
+        L.vm_synth_render.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_double,
+                                      C.c_double, C.c_uint64, C.c_double, C.c_void_p, C.c_void_p, C.c_double,
+                                      C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Workload:
+    """A synthetic sequence plus the map geometry it is fused into."""
+    name: str
+    seed: int
+    width: int
+    height: int
+    K: tuple  # fx, fy, cx, cy
+    n_frames: int
+    trajectory: int  # 0 circular, 1 random walk
+    depth_u16: bool
+    depth_scale: float  # metres per depth unit
+    max_depth: float
+    noise: float = 0.0012
+    dropout: float = 0.01
+    n_spheres: int = 5
+    n_boxes: int = 3
+    room: tuple = (4.0, 4.0, 3.0)
+    voxel_size: float = 0.01
+    trunc: float = 0.03
+    n_bucket: int = 1 << 17
+    n_excess: int = 1 << 14
+    n_block: int = 1 << 16
+    voxel_format: int = 0
+    max_step: float = 0.02
+    max_rot: float = math.radians(1.0)
+    _scene: int = field(default=0, repr=False)
+    _poses: np.ndarray | None = field(default=None, repr=False)
+
+    @property
+    def n_entry(self) -> int:
+        return self.n_bucket + self.n_excess
+
+    def scene(self):
+        if not self._scene:
+            L = _synth_lib()
+            self._scene = L.vm_synth_create(self.seed, self.n_spheres, self.n_boxes, *self.room)
+        return self._scene
+
+    def poses(self, n: int | None = None) -> np.ndarray:
+        n = self.n_frames if n is None else n
+        if self._poses is None or len(self._poses) < n:
+            out = np.zeros((n, 16), np.float32)
+            _synth_lib().vm_synth_trajectory(self.scene(), self.trajectory, n, self.seed * 7919 + 11,
+                                             self.max_step, self.max_rot, _ptr(out))
+            self._poses = out
+        return self._poses[:n]
+
+    def frame(self, i: int, want_rgb: bool = True):
+        """(depth, rgb): depth is uint16 raw units or float32 metres; rgb uint8 HxWx3."""
+        pose = np.ascontiguousarray(self.poses(i + 1)[i])
+        K = np.asarray(self.K, np.float32)
+        H, W = self.height, self.width
+        rgb = np.zeros((H, W, 3), np.uint8) if want_rgb else None
+        if self.depth_u16:
+            d16 = np.zeros((H, W), np.uint16)
+            _synth_lib().vm_synth_render(self.scene(), _ptr(pose), W, H, _ptr(K), self.noise, self.dropout,
+                                         self.seed * 1000003 + i, self.max_depth, None, _ptr(d16),
+                                         1.0 / self.depth_scale, _ptr(rgb))
+            return d16, rgb
+        d = np.zeros((H, W), np.float32)
+        _synth_lib().vm_synth_render(self.scene(), _ptr(pose), W, H, _ptr(K), self.noise, self.dropout,
+                                     self.seed * 1000003 + i, self.max_depth, _ptr(d), None, 1000.0, _ptr(rgb))
+        return d, rgb
+
+    def depth_metres(self, depth) -> np.ndarray:
+        """Float metres exactly as the library converts them: float(raw) * scale."""
+        if self.depth_u16:
+            return depth.astype(np.float32) * np.float32(self.depth_scale)
+        return depth
+
+    def __del__(self):
+        if self._scene and _lib is not None:
+            _lib.vm_synth_destroy(self._scene)
+            self._scene = 0
+
+
+def room_small(seed: int = 1, **kw) -> Workload:
+    args = dict(name="room_small_320x240_1cm", seed=seed, width=320, height=240, K=(262.5, 262.5, 159.5, 119.5),
+                n_frames=50, trajectory=0, depth_u16=False, depth_scale=1.0, max_depth=5.0,
+                voxel_size=0.01, trunc=0.03, n_bucket=1 << 17, n_excess=1 << 14, n_block=1 << 16,
+                voxel_format=0)
+    args.update(kw)
+    return Workload(**args)
+
+
+def room_vga(seed: int = 2, **kw) -> Workload:
+    args = dict(name="room_vga_640x480_5mm", seed=seed, width=640, height=480, K=(525.0, 525.0, 319.5, 239.5),
+                n_frames=500, trajectory=1, depth_u16=True, depth_scale=1.0 / 1000.0, max_depth=4.0,
+                voxel_size=0.005, trunc=0.02, n_bucket=1 << 20, n_excess=1 << 17, n_block=1 << 20,
+                voxel_format=2)
+    args.update(kw)
+    return Workload(**args)
