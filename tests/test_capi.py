@@ -41,3 +41,18 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(_capi.MapDesc) == 36
     assert ctypes.sizeof(_capi.MapStats) == 32
     assert ctypes.sizeof(_capi.MapArrays) == 9 * 8
+
+
+def test_cpp_dropin_api_exported_and_linked():
+    import subprocess
+    from voxelmapping_amd import _capi
+    out = subprocess.run(["nm", "-DC", "--defined-only", _capi.LIB_PATH], capture_output=True, text=True).stdout
+    for sym in ("vmap::VoxelMapping::VoxelMapping(", "vmap::VoxelMapping::CreateMap(", "vmap::VoxelMapping::FuseDepth(",
+                "vmap::VoxelMapping::FuseDepthAndImage(", "vmap::VoxelMapping::RenderScene(",
+                "vmap::VoxelMapping::Polygonize(", "vmap::VoxelMapping::GetSurfacePoints(",
+                "vmap::VoxelMapping::reset("):
+        assert sym in out, sym
+    demo = os.path.join(ROOT, "tests", "cpp", "dropin_demo")
+    assert os.path.exists(demo), "run the build first"
+    ldd = subprocess.run(["ldd", demo], capture_output=True, text=True).stdout
+    assert "libvmapping_amd.so" in ldd and "not found" not in ldd.split("libvmapping_amd.so")[1].split("\n")[0]

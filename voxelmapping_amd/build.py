@@ -86,9 +86,25 @@ def build_synth(verbose: bool = False) -> str:
     return SYNTH
 
 
+DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_demo.cpp")
+DEMO_BIN = os.path.join(ROOT, "tests", "cpp", "dropin_demo")
+
+
+def build_dropin_demo(verbose: bool = False) -> str:
+    """C++ program using the drop-in headers, linked against the library."""
+    tmp = DEMO_BIN + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", DEMO_SRC, "-o", tmp, f"-L{PKG}", "-lvmapping_amd",
+          "-Wl,-rpath,$ORIGIN/../../voxelmapping_amd"])
+    os.replace(tmp, DEMO_BIN)
+    if verbose:
+        print("built", DEMO_BIN)
+    return DEMO_BIN
+
+
 def build_all(verbose: bool = False) -> None:
     build_synth(verbose)
     build_hip_library(verbose)
+    build_dropin_demo(verbose)
 
 
 if __name__ == "__main__":

@@ -487,6 +487,31 @@ int vm_map_upload(vm_map* m, int32_t which, const void* src, int64_t bytes) {
     int64_t n;
     if (int rc = array_of(m, which, &p, &n)) return rc;
     if (!src || bytes != n) return fail(VM_ERR_INVALID_ARG, "host buffer size does not match the array");
+    // reject state that would make a kernel index outside the map's arrays
+    const MapView& v = m->view;
+    const int maxOff = v.nEntry - v.nBucket;
+    if (which == 0 || which == 4) {
+        const HashEntry* e = (const HashEntry*)src;
+        for (int i = 0; i < v.nEntry; ++i) {
+            const bool ptrOk = e[i].ptr == -1 || (e[i].ptr >= 0 && e[i].ptr % kBlockVox == 0 &&
+                                                  e[i].ptr / kBlockVox < v.nBlock);
+            if (!ptrOk || e[i].offset < -1 || e[i].offset > maxOff)
+                return fail(VM_ERR_INVALID_ARG, "hash entry " + std::to_string(i) + " out of range");
+        }
+    } else if (which == 1) {
+        const int* h = (const int*)src;
+        for (int i = 0; i < v.nBlock; ++i)
+            if (h[i] < -1 || h[i] >= v.nBlock) return fail(VM_ERR_INVALID_ARG, "heap entry out of range");
+    } else if (which == 3) {
+        const int* c = (const int*)src;
+        if (c[kHeapPtr] < -1 || c[kHeapPtr] >= v.nBlock || c[kExcessPtr] < 0 || c[kExcessPtr] > maxOff ||
+            c[kVisibleCount] < 0 || c[kVisibleCount] > v.nEntry)
+            return fail(VM_ERR_INVALID_ARG, "counters out of range");
+    } else if (which == 5) {
+        const int4* b = (const int4*)src;
+        for (int i = 0; i < v.nBlock; ++i)
+            if (b[i].w < -1 || b[i].w >= v.nEntry) return fail(VM_ERR_INVALID_ARG, "block position out of range");
+    }
     VM_HIP(hipDeviceSynchronize(), "synchronize");
     VM_HIP(hipMemcpy(p, src, (size_t)n, hipMemcpyHostToDevice), "upload");
     if (which == 0) {

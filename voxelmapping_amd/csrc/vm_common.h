@@ -123,6 +123,14 @@ __host__ __device__ inline int local_index(int lx, int ly, int lz) { return (lz 
 
 __host__ __device__ inline int voxel_bytes(int fmt) { return fmt == kFmtRef ? 4 : fmt == kFmtRefRGB ? 6 : 12; }
 
+// Allocated block ids are [0, A) with A = nBlock - 1 - heapPtr (the k-th pop
+// of the stack yields block k); clamped so a corrupted counter cannot index
+// past the block arrays.
+__device__ inline int allocated_count(const int* ctr, int nBlock) {
+    const int a = nBlock - 1 - ctr[0];
+    return a < 0 ? 0 : (a > nBlock ? nBlock : a);
+}
+
 // ---------------------------------------------------------------------------
 // voxel storage traits
 

@@ -59,7 +59,7 @@ __global__ void k_rebuild_index(MapView m) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.nEntry) return;
     HashEntry e = m.table[i];
-    if (e.ptr >= 0) m.blockPos[e.ptr / kBlockVox] = make_int4(e.x, e.y, e.z, i);
+    if (e.ptr >= 0 && e.ptr / kBlockVox < m.nBlock) m.blockPos[e.ptr / kBlockVox] = make_int4(e.x, e.y, e.z, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_visible(MapView m, DepthImage d, ColorI
     __shared__ int s_blk[256];
     __shared__ int4 s_key[256];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int A = m.nBlock - 1 - m.ctr[kHeapPtr];
+    const int A = allocated_count(m.ctr, m.nBlock);
     long long visSum = 0;
     for (int chunk = blockIdx.x * 256; chunk < A; chunk += gridDim.x * 256) {
         const int b = chunk + threadIdx.x;

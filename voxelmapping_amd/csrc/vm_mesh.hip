@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_polygonize(MapView m, float* verts, floa
     __shared__ uint8_t s_ok[kHalo];
     __shared__ int s_nb[8];
     const int lane = threadIdx.x;
-    const int A = m.nBlock - 1 - m.ctr[kHeapPtr];
+    const int A = allocated_count(m.ctr, m.nBlock);
     const float vs = m.voxelSize;
     for (int b = blockIdx.x; b < A; b += gridDim.x) {
         const int4 key = m.blockPos[b];
