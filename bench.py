@@ -73,6 +73,112 @@ def cpu_baseline(w, seconds: float) -> dict:
                       f"({el:.1f} s, OpenMP integration/raycast, serial allocation)"}
 
 
+def batched_scenes(n=16, seed=4):
+    """16 seeded scenes: resolution from {160x120, 320x240, 640x480}, voxel
+    size from {4, 6, 10} mm (truncation 4 voxels), 2^16..2^20 buckets by image
+    size and voxel size, uint16 mm depth, float TSDF, random-walk trajectory."""
+    from voxelmapping_amd import synth
+    rng = np.random.default_rng(seed)
+    res = [(160, 120), (320, 240), (640, 480)]
+    out = []
+    for i in range(n):
+        W, H = res[rng.integers(3)]
+        vs = [0.004, 0.006, 0.010][rng.integers(3)]
+        f = 525.0 * W / 640
+        lb = {160: 16, 320: 18, 640: 20}[W] - (1 if vs >= 0.010 else 0) + (1 if vs <= 0.004 else 0)
+        lb = max(16, min(20, lb))
+        out.append(synth.room_vga(seed=100 + i, width=W, height=H, K=(f, f, W / 2 - 0.5, H / 2 - 0.5),
+                                  voxel_size=vs, trunc=4 * vs, n_bucket=1 << lb, n_excess=1 << (lb - 3),
+                                  n_block=1 << min(lb, 18), voxel_format=2, name=f"scene{i}_{W}x{H}_{vs * 1000:.0f}mm"))
+    return out
+
+
+def batched_bench(warmup: int, steps: int) -> dict:
+    """Plain per-scene loop vs one batched step (autotuned, hipGraph) over the
+    same frames of the 16-scene workload; aggregate frames/s of both."""
+    import torch
+    from voxelmapping_amd import _capi
+    from voxelmapping_amd.mapping import Batch, DeviceMap, intrinsics, pose_colmajor
+    L = _capi.lib()
+    ws = batched_scenes()
+    nfr = warmup + steps
+    dev = torch.device("cuda:0")
+    maps, specs, plain_args = [], [], []
+    for w in ws:
+        d = np.stack([w.frame(i, want_rgb=False)[0] for i in range(nfr)])
+        dt = torch.from_numpy(d.view(np.int16)).to(dev).view(torch.uint16)
+        m = DeviceMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, w.voxel_format)
+        maps.append(m)
+        specs.append(dict(map=m, K=w.K, depth=dt, depth_scale=w.depth_scale, poses=w.poses(nfr)))
+        plain_args.append((w, dt, [pose_colmajor(w.poses(nfr)[i]) for i in range(nfr)], intrinsics(w.K)))
+    b = Batch(specs)
+    stream = torch.cuda.Stream()
+    sp = C.c_void_p(stream.cuda_stream)
+
+    def plain_step(f):
+        for (w, dt, poses, K), m, vm, nm in zip(plain_args, maps, b.vmaps, b.nmaps):
+            W, H = w.width, w.height
+            _capi.check(L.vm_fuse_depth(m.handle, C.c_void_p(dt[f].data_ptr()), W * 2, W, H, 1,
+                                        C.c_float(w.depth_scale), None, 0, K, poses[f], sp, None))
+            _capi.check(L.vm_render_scene(m.handle, C.c_void_p(vm.data_ptr()), W * 16, W, H, K, poses[f], sp))
+            _capi.check(L.vm_compute_normals(C.c_void_p(vm.data_ptr()), W * 16, C.c_void_p(nm.data_ptr()), W * 16,
+                                             W, H, sp))
+
+    def sums():
+        return sum(m.stats(stream)["sum_visible"] for m in maps)
+
+    # plain loop
+    with torch.cuda.stream(stream):
+        for f in range(warmup):
+            plain_step(f)
+        stream.synchronize()
+        v0 = sums()
+        t0 = time.perf_counter()
+        for f in range(warmup, nfr):
+            plain_step(f)
+        stream.synchronize()
+        t_plain = time.perf_counter() - t0
+        v_plain = sums() - v0
+        # batched: same maps from scratch, autotune inside the warmup frames
+        for m in maps:
+            m.reset(stream)
+        b.set_frame(0, stream)
+        tune_steps = 1
+        best = b.tune(steps=tune_steps, stream=stream)  # 18 candidates x 1 step
+        done = 18 * tune_steps
+        if done > warmup:
+            raise SystemExit("warmup must cover the autotuner's steps")
+        for _ in range(warmup - done):
+            b.step(stream)
+        b.capture(stream)
+        stream.synchronize()
+        v0 = sums()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.step(stream)
+        stream.synchronize()
+        t_batch = time.perf_counter() - t0
+        v_batch = sums() - v0
+    n = len(ws)
+    P_total = sum(w.width * w.height for w in ws)
+    bytes_frame_pixels = P_total * (B_DEPTH + B_VMAP + B_NMAP)  # no colour in this workload
+    per_vis = 512 * (2 * S_VOXEL + S_RAY) + S_ENTRY
+    batch_bytes = steps * bytes_frame_pixels + v_batch * per_vis
+    return {
+        "workload": "16 scenes, 160x120..640x480, 4/6/10 mm, 2^16..2^20 buckets, uint16 depth, float TSDF",
+        "scenes": [w.name for w in ws],
+        "frames": [warmup, nfr],
+        "value": round(n * steps / t_batch, 2), "unit": "frames/s (all scenes)",
+        "ms_per_step": round(1000 * t_batch / steps, 4),
+        "plain_loop_value": round(n * steps / t_plain, 2),
+        "plain_loop_ms_per_step": round(1000 * t_plain / steps, 4),
+        "speedup_vs_plain_loop": round(t_plain / t_batch, 3),
+        "hbm_frac": round(batch_bytes / t_batch / 1e9 / HBM_PEAK_GBS, 4),
+        "visible_blocks_equal": bool(v_plain == v_batch),
+        "tuning": best, "graph": True,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,7 +186,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 16-scene batched comparison")
+    ap.add_argument("--batched-warmup", type=int, default=20)
+    ap.add_argument("--batched-steps", type=int, default=180)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (scales rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 64 << 20  # floats
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(scenes shard per GPU with no exchange; see the design notes)", file=sys.stderr)
@@ -174,6 +295,8 @@ def main():
         "stage_avg_us": {n: round(v, 2) for n, v in stage_avg_us.items()},
         "status": s1["status"],
     }
+    if not args.no_batched:
+        out["batched"] = batched_bench(args.batched_warmup, args.batched_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth.room_vga(), args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -175,6 +175,65 @@ int vm_surface_points(const float* verts, float* points, int64_t n_triangles, vo
 int vm_map_download(vm_map* map, int32_t which, void* host_dst, int64_t bytes);
 int vm_map_upload(vm_map* map, int32_t which, const void* host_src, int64_t bytes);
 
+/* ---------------------------------------------------------------------------
+ * Multi-scene batches (no counterpart in the reference: its API fuses one
+ * scene per call). Many independent maps of uneven image size, voxel size and
+ * table size are advanced in lockstep: one step runs allocation, visibility,
+ * integration, raycast and normals for every scene at that scene's current
+ * frame, with each stage's work from all scenes in one launch. Per-frame
+ * inputs live in device memory (frames stored at a fixed stride, poses
+ * uploaded at creation) and the frame index is a device counter, so a step
+ * can be captured once into a hipGraph and replayed. Per scene the results
+ * equal vm_fuse_depth + vm_render_scene + vm_compute_normals at that frame.
+ * All maps of a batch share one voxel format, one depth format and colour
+ * (all scenes or none); at most 64 scenes; a map belongs to one batch. */
+typedef struct vm_batch vm_batch;
+
+typedef struct vm_batch_scene {
+    vm_map* map;
+    int32_t width, height;
+    float K[4];
+    int32_t depth_format;
+    float depth_scale;
+    const void* depth_frames;   /* device: frame f at depth_frames + f * depth_frame_stride */
+    int64_t depth_pitch;
+    int64_t depth_frame_stride;
+    const void* rgb_frames;     /* device or NULL: 3 bytes per pixel */
+    int64_t rgb_pitch;
+    int64_t rgb_frame_stride;
+    const float* poses;         /* host: n_frames column-major camera-to-world 4x4 */
+    int32_t n_frames;           /* steps past the last frame leave the scene unchanged */
+    float* vmap;                /* device outputs, float4 per pixel */
+    int64_t vmap_pitch;
+    float* nmap;
+    int64_t nmap_pitch;
+} vm_batch_scene;
+
+/* launch shape of a batch step (chosen by vm_batch_tune) */
+typedef struct vm_batch_tuning {
+    int32_t ray_tile_width;       /* 8 (8x8 rays per wave), 16 (16x4) or 32 (32x2) */
+    int32_t xcd_bands;            /* 1: give each XCD a contiguous range of ray tiles */
+    int32_t visible_wg_per_cu;    /* visibility workgroups per compute unit */
+    int32_t integrate_wg_per_cu;  /* integration workgroups (4 waves) per compute unit */
+} vm_batch_tuning;
+
+int vm_batch_create(const vm_batch_scene* scenes, int32_t n_scenes, vm_batch** out_batch);
+int vm_batch_destroy(vm_batch* batch);
+/* device-side frame index used by the next step (stream-ordered) */
+int vm_batch_set_frame(vm_batch* batch, int32_t frame, void* stream);
+int vm_batch_get_frame(vm_batch* batch, void* stream, int32_t* out_frame);
+/* one step for all scenes, then frame += 1; replays the captured graph if any */
+int vm_batch_step(vm_batch* batch, void* stream);
+/* capture one step into a hipGraph (stream must not be NULL) */
+int vm_batch_capture(vm_batch* batch, void* stream);
+int vm_batch_get_tuning(vm_batch* batch, vm_batch_tuning* out_tuning);
+/* changes the launch shape; drops a captured graph */
+int vm_batch_set_tuning(vm_batch* batch, const vm_batch_tuning* tuning);
+/* Autotuner: times every candidate launch shape over `steps` steps each (the
+ * steps fuse frames like any step) and applies the fastest; drops a captured
+ * graph. */
+int vm_batch_tune(vm_batch* batch, void* stream, int32_t steps, vm_batch_tuning* out_best);
+
 #ifdef __cplusplus
 }
 #endif

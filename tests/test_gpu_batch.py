@@ -1,0 +1,155 @@
+"""Multi-scene batches: per scene, a batched step equals the plain pipeline
+(vm_fuse_depth + vm_render_scene + vm_compute_normals) at that frame, bit
+for bit -- eager, after autotuning, and replayed from a captured hipGraph.
+One scene is also checked against the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from parity import block_map, compare_maps
+
+pytestmark = pytest.mark.gpu
+
+
+def scenes_spec():
+    from voxelmapping_amd import synth
+    return [
+        synth.room_small(seed=1, n_frames=12, voxel_format=2),
+        synth.room_small(seed=7, n_frames=12, width=160, height=120, K=(131.25, 131.25, 79.5, 59.5),
+                         voxel_size=0.006, trunc=0.024, n_bucket=1 << 16, n_excess=1 << 13, n_block=1 << 15,
+                         voxel_format=2),
+        synth.room_small(seed=9, n_frames=12, voxel_size=0.004, trunc=0.016, n_bucket=1 << 18, n_excess=1 << 15,
+                         n_block=1 << 16, voxel_format=2),
+    ]
+
+
+def new_map(w):
+    from voxelmapping_amd import DeviceMap
+    return DeviceMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, w.voxel_format)
+
+
+def frames_on_gpu(w, n):
+    import torch
+    d = np.stack([w.frame(i)[0] for i in range(n)])
+    c = np.stack([w.frame(i)[1] for i in range(n)])
+    return torch.from_numpy(d).cuda(), torch.from_numpy(c).cuda()
+
+
+def build_batch(ws, n_frames):
+    from voxelmapping_amd.mapping import Batch
+    maps, specs = [], []
+    for w in ws:
+        m = new_map(w)
+        d, c = frames_on_gpu(w, n_frames)
+        maps.append(m)
+        specs.append(dict(map=m, K=w.K, depth=d, rgb=c, poses=w.poses(n_frames)))
+    return Batch(specs), maps
+
+
+def plain_run(w, frames):
+    import torch
+    from voxelmapping_amd import computeNormal
+    m = new_map(w)
+    vm = torch.zeros((w.height, w.width, 4), dtype=torch.float32, device="cuda")
+    nm = torch.zeros_like(vm)
+    for i in frames:
+        d, c = w.frame(i)
+        T = w.poses(i + 1)[i]
+        m.fuse(torch.from_numpy(d).cuda(), w.K, T, rgb=torch.from_numpy(c).cuda())
+        m.render_scene(vm, w.K, T)
+        computeNormal(vm, nm)
+    torch.cuda.synchronize()
+    return m, vm, nm
+
+
+def same_maps(a, b):
+    ta, tb = a.hash_table(), b.hash_table()
+    ka, kb = block_map(ta), block_map(tb)
+    assert set(ka) == set(kb)
+    va, vb = a.download(2).reshape(-1, 12), b.download(2).reshape(-1, 12)
+    for k in ka:
+        assert np.array_equal(va[ka[k]:ka[k] + 512], vb[kb[k]:kb[k] + 512]), k
+
+
+def same_images(x, y):
+    x, y = x.cpu().numpy(), y.cpu().numpy()
+    assert np.array_equal(np.isnan(x), np.isnan(y))
+    assert np.array_equal(np.nan_to_num(x, nan=7.0), np.nan_to_num(y, nan=7.0))
+
+
+def test_batch_equals_plain_eager():
+    import torch
+    ws = scenes_spec()
+    b, maps = build_batch(ws, 6)
+    for _ in range(6):
+        b.step()
+    torch.cuda.synchronize()
+    assert b.frame() == 6
+    for i, w in enumerate(ws):
+        m, vm, nm = plain_run(w, range(6))
+        same_maps(maps[i], m)
+        same_images(b.vmaps[i], vm)
+        same_images(b.nmaps[i][:-1, :-1], nm[:-1, :-1])
+        assert maps[i].stats()["sum_visible"] == m.stats()["sum_visible"]
+
+
+def test_batch_graph_replay_and_tuning():
+    import torch
+    ws = scenes_spec()
+    # tune on a throw-away batch, apply the result to the checked one
+    tb, _ = build_batch(ws, 8)
+    best = tb.tune(steps=1)
+    assert best["ray_tile_width"] in (8, 16, 32)
+    tb.close()
+    b, maps = build_batch(ws, 8)
+    b.set_tuning(**best)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b.step(stream=s)          # frame 0 eagerly
+        b.capture(stream=s)       # frames 1.. from the graph
+        for _ in range(4):
+            b.step(stream=s)
+            s.synchronize()
+            torch.cuda.synchronize()  # eager work between replays
+    s.synchronize()
+    assert b.frame(stream=s) == 5
+    for i, w in enumerate(ws):
+        m, vm, nm = plain_run(w, range(5))
+        same_maps(maps[i], m)
+        same_images(b.vmaps[i], vm)
+
+
+def test_batch_scene_against_oracle(oracle_threads):
+    import torch
+    ws = scenes_spec()[1:2]
+    b, maps = build_batch(ws, 4)
+    for _ in range(4):
+        b.step()
+    torch.cuda.synchronize()
+    w = ws[0]
+    o = O.OracleMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, 2)
+    for i in range(4):
+        d, c = w.frame(i)
+        o.fuse(d, w.K, w.poses(i + 1)[i], rgb=c)
+    compare_maps(maps[0], o, 2)
+    ov = o.raycast(w.width, w.height, w.K, w.poses(4)[3], init=0.0)
+    gv = b.vmaps[0].cpu().numpy()
+    assert np.array_equal(np.isnan(gv[..., 0]), np.isnan(ov[..., 0]))
+
+
+def test_batch_frames_exhausted_and_errors():
+    import torch
+    from voxelmapping_amd import VMError
+    from voxelmapping_amd.mapping import Batch
+    ws = scenes_spec()[:2]
+    b, maps = build_batch(ws, 2)
+    for _ in range(4):  # two steps past the last frame
+        b.step()
+    torch.cuda.synchronize()
+    for i, w in enumerate(ws):
+        m, vm, _ = plain_run(w, range(2))
+        same_maps(maps[i], m)
+        assert maps[i].stats()["frames_fused"] == 2
+    with pytest.raises(VMError):
+        Batch([dict(map=maps[0], K=ws[0].K, depth=torch.zeros((1, 4, 4), device="cuda"), poses=np.eye(4)[None]),
+               dict(map=maps[0], K=ws[0].K, depth=torch.zeros((1, 4, 4), device="cuda"), poses=np.eye(4)[None])])

@@ -70,6 +70,38 @@ class MapStats(C.Structure):
     ]
 
 
+class BatchScene(C.Structure):
+    _fields_ = [
+        ("map", C.c_void_p),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("K", C.c_float * 4),
+        ("depth_format", C.c_int32),
+        ("depth_scale", C.c_float),
+        ("depth_frames", C.c_void_p),
+        ("depth_pitch", C.c_int64),
+        ("depth_frame_stride", C.c_int64),
+        ("rgb_frames", C.c_void_p),
+        ("rgb_pitch", C.c_int64),
+        ("rgb_frame_stride", C.c_int64),
+        ("poses", C.c_void_p),
+        ("n_frames", C.c_int32),
+        ("vmap", C.c_void_p),
+        ("vmap_pitch", C.c_int64),
+        ("nmap", C.c_void_p),
+        ("nmap_pitch", C.c_int64),
+    ]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [
+        ("ray_tile_width", C.c_int32),
+        ("xcd_bands", C.c_int32),
+        ("visible_wg_per_cu", C.c_int32),
+        ("integrate_wg_per_cu", C.c_int32),
+    ]
+
+
 _P = C.c_void_p
 _F4 = C.POINTER(C.c_float)
 _U32P = C.POINTER(C.c_uint32)
@@ -101,6 +133,15 @@ SIGNATURES = {
     "vm_surface_points": (C.c_int, [_P, _P, C.c_int64, _P]),
     "vm_map_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "vm_map_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "vm_batch_create": (C.c_int, [C.POINTER(BatchScene), C.c_int32, C.POINTER(_P)]),
+    "vm_batch_destroy": (C.c_int, [_P]),
+    "vm_batch_set_frame": (C.c_int, [_P, C.c_int32, _P]),
+    "vm_batch_get_frame": (C.c_int, [_P, _P, C.POINTER(C.c_int32)]),
+    "vm_batch_step": (C.c_int, [_P, _P]),
+    "vm_batch_capture": (C.c_int, [_P, _P]),
+    "vm_batch_get_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
+    "vm_batch_set_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
+    "vm_batch_tune": (C.c_int, [_P, _P, C.c_int32, C.POINTER(BatchTuning)]),
 }
 
 _lib = None

@@ -1,0 +1,52 @@
+// Device-side descriptors of the multi-scene pipeline (vm_batch.hip).
+#pragma once
+
+#include "vm_common.h"
+
+namespace vmamd {
+
+// one scene of a batch, in device memory
+struct BatchScene {
+    MapView m;
+    Intrinsics K;
+    DepthImage d;             // data = frame 0; frame f at data + f * depthFrameStride
+    ColorImage c;             // data = frame 0 or null
+    int64_t depthFrameStride;
+    int64_t rgbFrameStride;
+    const Affine* poses;      // [nFrames] camera-to-world
+    const Affine* posesInv;   // [nFrames] world-to-camera
+    int nFrames;
+    float* vmap;
+    int64_t vpitch;
+    float* nmap;
+    int64_t npitch;
+    float vsInv, rtStep;      // (float)(1.0 / voxelSize), trunc / voxelSize
+    int tile16First;          // first 16x16 tile of this scene in the flat tile space
+    int tilesX16;
+    int rayTileFirst;         // first ray tile of this scene
+};
+
+struct BatchView {
+    const BatchScene* scenes;  // device array
+    int nScenes;
+    int* frame;                // device-side frame index
+    int totalTile16;
+    int totalRayTiles;
+    int rayTilesPerXcd;
+    int maxChunks;             // sum over scenes of ceil(nBlock / 256)
+    int maxVisible;            // sum over scenes of min(nBlock, nEntry)
+};
+
+// launch shape picked by the autotuner
+struct BatchTuning {
+    int rayTileW;           // ray tile width: 8 (8x8), 16 (16x4) or 32 (32x2)
+    int xcdBands;           // remap ray tiles into per-XCD contiguous bands
+    int visibleWgPerCu;     // visibility workgroups per compute unit
+    int integrateWgPerCu;   // integration workgroups (4 waves each) per CU
+};
+
+hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, int df, bool color, hipStream_t s);
+hipError_t batch_set_frame(int* frame, int v, hipStream_t s);
+int batch_max_scenes();
+
+}  // namespace vmamd

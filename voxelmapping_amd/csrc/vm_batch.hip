@@ -1,0 +1,205 @@
+// Multi-scene fuse + raycast: many independent maps of uneven image size,
+// voxel size and table size advanced in lockstep. One step runs five
+// kernels over the union of all scenes' work (allocation tiles, allocated
+// blocks, visible blocks, ray tiles, normal tiles), so small scenes share the
+// machine instead of each leaving most of it idle; the step can be captured
+// into a hipGraph and replayed, since every per-frame input (depth, colour,
+// pose) is found on the device from a device-side frame counter.
+//
+// Per scene the arithmetic is exactly the single-map pipeline's
+// (vm_dev_fusion.h, vm_dev_ray.h), so batched and plain results are
+// identical.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vm_common.h"
+#include "vm_dev_fusion.h"
+#include "vm_dev_ray.h"
+#include "vm_launch.h"
+#include "vm_batch.h"
+
+namespace vmamd {
+
+namespace {
+
+constexpr int kMaxScenes = 64;
+
+// scene of a flat work index: the last scene whose first index is <= i
+__device__ inline int scene_of(const int* first, int n, int i) {
+    int s = 0;
+    for (int k = 1; k < n; ++k) s = (first[k] <= i) ? k : s;
+    return s;
+}
+
+__global__ __launch_bounds__(256) void kb_allocate(BatchView B) {
+    __shared__ int s_first[kMaxScenes];
+    for (int k = threadIdx.x; k < B.nScenes; k += blockDim.x) s_first[k] = B.scenes[k].tile16First;
+    __syncthreads();
+    const int s = scene_of(s_first, B.nScenes, blockIdx.x);
+    const BatchScene& S = B.scenes[s];
+    const int f = *B.frame;
+    if (f >= S.nFrames) return;  // past its last frame a scene is left as it is
+    const int lt = blockIdx.x - S.tile16First;
+    if (lt == 0 && threadIdx.x == 0) {
+        S.m.ctr[kVisibleCount] = 0;
+        S.m.ctr[kFramesFused] += 1;
+    }
+    const int x = (lt % S.tilesX16) * 16 + (threadIdx.x & 15);
+    const int y = (lt / S.tilesX16) * 16 + (threadIdx.x >> 4);
+    if (x >= S.d.width || y >= S.d.height) return;
+    DepthImage d = S.d;
+    d.data += (int64_t)f * S.depthFrameStride;
+    if (d.format == kDepthU16) allocate_pixel<kDepthU16>(S.m, d, S.K, S.poses[f], x, y);
+    else allocate_pixel<kDepthF32>(S.m, d, S.K, S.poses[f], x, y);
+}
+
+// chunks of 256 allocated blocks of every scene, concatenated
+__global__ __launch_bounds__(256) void kb_visible(BatchView B) {
+    __shared__ VisibleShared sh;
+    __shared__ int s_first[kMaxScenes + 1];
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < B.nScenes; ++k) {
+            s_first[k] = acc;
+            acc += (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + 255) / 256;
+        }
+        s_first[B.nScenes] = acc;
+    }
+    __syncthreads();
+    const int total = s_first[B.nScenes];
+    const int f = *B.frame;
+    for (int c = blockIdx.x; c < total; c += gridDim.x) {
+        const int s = scene_of(s_first, B.nScenes, c);
+        const BatchScene& S = B.scenes[s];
+        if (f >= S.nFrames) continue;
+        const int A = allocated_count(S.m.ctr, S.m.nBlock);
+        const DepthImage d{};
+        const ColorImage col{};
+        visible_chunk<false, kFmtRef, kDepthF32, false>(S.m, d, col, S.K, S.posesInv[f], S.d.width, S.d.height,
+                                                         (c - s_first[s]) * 256, A, sh);
+    }
+}
+
+// visible blocks of every scene, concatenated; one wave per block
+template <int FMT, int DF, bool COLOR>
+__global__ __launch_bounds__(256) void kb_integrate(BatchView B) {
+    __shared__ int s_first[kMaxScenes + 1];
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < B.nScenes; ++k) {
+            s_first[k] = acc;
+            acc += B.scenes[k].m.ctr[kVisibleCount];
+        }
+        s_first[B.nScenes] = acc;
+    }
+    __syncthreads();
+    const int total = s_first[B.nScenes];
+    const int f = *B.frame;
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w < B.nScenes && lane == 0 && f < B.scenes[w].nFrames) {
+        const int V = s_first[w + 1] - s_first[w];
+        if (V > 0) atomicAdd((unsigned long long*)&B.scenes[w].m.ctr[kSumVisibleLo], (unsigned long long)V);
+    }
+    for (int i = w; i < total; i += gridDim.x * 4) {
+        const int s = scene_of(s_first, B.nScenes, i);
+        const BatchScene& S = B.scenes[s];
+        if (f >= S.nFrames) continue;
+        DepthImage d = S.d;
+        d.data += (int64_t)f * S.depthFrameStride;
+        ColorImage col = S.c;
+        if (COLOR) col.data += (int64_t)f * S.rgbFrameStride;
+        const HashEntry e = S.m.visible[i - s_first[s]];
+        integrate_block<FMT, DF, COLOR>(S.m, d, col, S.K, S.posesInv[f], e.ptr / kBlockVox,
+                                        make_int4(e.x, e.y, e.z, 0), lane);
+    }
+}
+
+// ray tiles of every scene, concatenated; tile shape tw x (64 / tw); with
+// xcdBands the flat tile index is remapped so each XCD gets a contiguous range
+template <int FMT>
+__global__ __launch_bounds__(64) void kb_raycast(BatchView B, int tw, int xcdBands) {
+    __shared__ int s_first[kMaxScenes];
+    for (int k = threadIdx.x; k < B.nScenes; k += blockDim.x) s_first[k] = B.scenes[k].rayTileFirst;
+    __syncthreads();
+    const int g = blockIdx.x;
+    const int t = xcdBands ? (g & 7) * B.rayTilesPerXcd + (g >> 3) : g;
+    if (t >= B.totalRayTiles) return;
+    const int s = scene_of(s_first, B.nScenes, t);
+    const BatchScene& S = B.scenes[s];
+    const int f = *B.frame;
+    if (f >= S.nFrames) return;
+    const int th = 64 / tw;
+    const int tilesX = (S.d.width + tw - 1) / tw;
+    const int lt = t - S.rayTileFirst;
+    const int x = (lt % tilesX) * tw + (threadIdx.x % tw);
+    const int y = (lt / tilesX) * th + (threadIdx.x / tw);
+    if (x >= S.d.width || y >= S.d.height) return;
+    if (S.m.ctr[kVisibleCount] == 0) return;  // the facade's gate (vmapping.cpp:87-88)
+    float* out = (float*)((uint8_t*)S.vmap + (int64_t)y * S.vpitch) + x * 4;
+    raycast_pixel<FMT>(S.m, out, x, y, S.K, S.poses[f], S.posesInv[f], S.vsInv, S.rtStep);
+}
+
+__global__ __launch_bounds__(256) void kb_normals(BatchView B) {
+    __shared__ int s_first[kMaxScenes];
+    for (int k = threadIdx.x; k < B.nScenes; k += blockDim.x) s_first[k] = B.scenes[k].tile16First;
+    __syncthreads();
+    const int s = scene_of(s_first, B.nScenes, blockIdx.x);
+    const BatchScene& S = B.scenes[s];
+    if (*B.frame >= S.nFrames) return;
+    const int lt = blockIdx.x - S.tile16First;
+    const int x = (lt % S.tilesX16) * 16 + (threadIdx.x & 15);
+    const int y = (lt / S.tilesX16) * 16 + (threadIdx.x >> 4);
+    const int W = S.d.width, H = S.d.height;
+    if (x >= W - 1 || y >= H - 1) return;
+    normal_pixel(S.vmap, S.vpitch, S.nmap, S.npitch, W, H, x, y);
+}
+
+__global__ void kb_advance(int* frame) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *frame += 1;
+}
+
+__global__ void kb_set_frame(int* frame, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *frame = v;
+}
+
+}  // namespace
+
+hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, int df, bool color, hipStream_t s) {
+    const int cu = num_compute_units();
+    hipLaunchKernelGGL(kb_allocate, dim3(B.totalTile16), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(kb_visible, dim3(std::max(1, std::min(cu * t.visibleWgPerCu, B.maxChunks))), dim3(256), 0,
+                       s, B);
+    const dim3 gi(std::max(1, std::min(cu * t.integrateWgPerCu, (B.maxVisible + 3) / 4)));
+#define VM_INTEGRATE(F, D, C) hipLaunchKernelGGL((kb_integrate<F, D, C>), gi, dim3(256), 0, s, B)
+    if (fmt == kFmtRef) {
+        if (df == kDepthU16) VM_INTEGRATE(kFmtRef, kDepthU16, false); else VM_INTEGRATE(kFmtRef, kDepthF32, false);
+    } else if (fmt == kFmtRefRGB) {
+        if (df == kDepthU16) { if (color) VM_INTEGRATE(kFmtRefRGB, kDepthU16, true); else VM_INTEGRATE(kFmtRefRGB, kDepthU16, false); }
+        else { if (color) VM_INTEGRATE(kFmtRefRGB, kDepthF32, true); else VM_INTEGRATE(kFmtRefRGB, kDepthF32, false); }
+    } else {
+        if (df == kDepthU16) { if (color) VM_INTEGRATE(kFmtF32, kDepthU16, true); else VM_INTEGRATE(kFmtF32, kDepthU16, false); }
+        else { if (color) VM_INTEGRATE(kFmtF32, kDepthF32, true); else VM_INTEGRATE(kFmtF32, kDepthF32, false); }
+    }
+#undef VM_INTEGRATE
+    const int tiles = t.xcdBands ? B.rayTilesPerXcd * 8 : B.totalRayTiles;
+    if (fmt == kFmtRef) hipLaunchKernelGGL(kb_raycast<kFmtRef>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
+    else if (fmt == kFmtRefRGB) hipLaunchKernelGGL(kb_raycast<kFmtRefRGB>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
+    else hipLaunchKernelGGL(kb_raycast<kFmtF32>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
+    hipLaunchKernelGGL(kb_normals, dim3(B.totalTile16), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(kb_advance, dim3(1), dim3(64), 0, s, B.frame);
+    return hipGetLastError();
+}
+
+hipError_t batch_set_frame(int* frame, int v, hipStream_t s) {
+    hipLaunchKernelGGL(kb_set_frame, dim3(1), dim3(64), 0, s, frame, v);
+    return hipGetLastError();
+}
+
+int batch_max_scenes() { return kMaxScenes; }
+
+}  // namespace vmamd

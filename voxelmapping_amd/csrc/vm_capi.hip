@@ -8,7 +8,9 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
+#include "vm_batch.h"
 #include "vm_common.h"
 #include "vm_launch.h"
 #include "vmapping_capi.h"
@@ -523,6 +525,286 @@ int vm_map_upload(vm_map* m, int32_t which, const void* src, int64_t bytes) {
     m->epoch++;
     m->visValid = false;
     return VM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// multi-scene batches
+
+struct vm_batch {
+    std::vector<vm_map*> maps;
+    std::vector<vm_batch_scene> desc;
+    std::vector<BatchScene> host;
+    BatchScene* dScenes = nullptr;
+    Affine* dPoses = nullptr;  // per scene: nFrames camToWorld, then nFrames worldToCam
+    int* dFrame = nullptr;
+    BatchView view{};
+    BatchTuning tuning{8, 1, 4, 8};
+    int fmt = 0, df = 0;
+    bool color = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void batch_drop_graph(vm_batch* b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// (re)build the per-scene device descriptors for the current tuning
+int batch_upload(vm_batch* b) {
+    const int n = (int)b->maps.size();
+    int tile16 = 0, ray = 0, chunks = 0, vis = 0;
+    const int tw = b->tuning.rayTileW, th = 64 / tw;
+    for (int i = 0; i < n; ++i) {
+        BatchScene& S = b->host[i];
+        const vm_batch_scene& d = b->desc[i];
+        S.tile16First = tile16;
+        S.tilesX16 = (d.width + 15) / 16;
+        tile16 += S.tilesX16 * ((d.height + 15) / 16);
+        S.rayTileFirst = ray;
+        ray += ((d.width + tw - 1) / tw) * ((d.height + th - 1) / th);
+        chunks += (S.m.nBlock + 255) / 256;
+        vis += S.m.nBlock < S.m.nEntry ? S.m.nBlock : S.m.nEntry;
+    }
+    b->view.scenes = b->dScenes;
+    b->view.nScenes = n;
+    b->view.frame = b->dFrame;
+    b->view.totalTile16 = tile16;
+    b->view.totalRayTiles = ray;
+    b->view.rayTilesPerXcd = (ray + 7) / 8;
+    b->view.maxChunks = chunks;
+    b->view.maxVisible = vis;
+    VM_HIP(hipMemcpy(b->dScenes, b->host.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice), "batch upload");
+    return VM_OK;
+}
+
+void batch_touch_maps(vm_batch* b) {
+    for (vm_map* m : b->maps) {
+        m->epoch++;
+        m->visValid = false;
+        m->visCountZeroed = false;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
+    if (!scenes || !out) return fail(VM_ERR_INVALID_ARG, "scenes and out_batch must not be NULL");
+    *out = nullptr;
+    if (n <= 0 || n > batch_max_scenes())
+        return fail(VM_ERR_INVALID_ARG, "scene count must be in [1, " + std::to_string(batch_max_scenes()) + "]");
+    const int fmt = scenes[0].map ? scenes[0].map->view.fmt : 0;
+    const int df = scenes[0].depth_format;
+    const bool color = scenes[0].rgb_frames != nullptr;
+    size_t nPoses = 0;
+    for (int i = 0; i < n; ++i) {
+        const vm_batch_scene& d = scenes[i];
+        const std::string who = "scene " + std::to_string(i) + ": ";
+        if (!d.map) return fail(VM_ERR_INVALID_ARG, who + "map is NULL");
+        for (int j = 0; j < i; ++j)
+            if (scenes[j].map == d.map) return fail(VM_ERR_INVALID_ARG, who + "map used twice");
+        if (d.map->view.fmt != fmt) return fail(VM_ERR_INVALID_ARG, who + "all maps must share a voxel format");
+        if (d.depth_format != df) return fail(VM_ERR_INVALID_ARG, who + "all scenes must share a depth format");
+        if ((d.rgb_frames != nullptr) != color) return fail(VM_ERR_INVALID_ARG, who + "colour for all or none");
+        if (color && fmt == VM_VOXEL_I16_U8) return fail(VM_ERR_INVALID_ARG, who + "format stores no colour");
+        if (df != VM_DEPTH_F32 && df != VM_DEPTH_U16) return fail(VM_ERR_INVALID_ARG, who + "unknown depth format");
+        if (int rc = check_image(d.depth_frames, d.depth_pitch, d.width, d.height, df == VM_DEPTH_F32 ? 4 : 2,
+                                 (who + "depth").c_str()))
+            return rc;
+        if (color)
+            if (int rc = check_image(d.rgb_frames, d.rgb_pitch, d.width, d.height, 3, (who + "rgb").c_str()))
+                return rc;
+        if (int rc = check_image(d.vmap, d.vmap_pitch, d.width, d.height, 16, (who + "vmap").c_str())) return rc;
+        if (int rc = check_image(d.nmap, d.nmap_pitch, d.width, d.height, 16, (who + "nmap").c_str())) return rc;
+        if (d.vmap_pitch % 16 || d.nmap_pitch % 16) return fail(VM_ERR_INVALID_ARG, who + "misaligned map pitch");
+        if (d.n_frames <= 0 || !d.poses) return fail(VM_ERR_INVALID_ARG, who + "needs frames and poses");
+        if (d.depth_frame_stride < d.depth_pitch * d.height || (color && d.rgb_frame_stride < d.rgb_pitch * d.height))
+            return fail(VM_ERR_INVALID_ARG, who + "frame stride smaller than a frame");
+        if (int rc = check_pose_k(d.K, d.poses)) return rc;
+        nPoses += 2 * (size_t)d.n_frames;
+    }
+    vm_batch* b = new vm_batch;
+    b->fmt = fmt;
+    b->df = df;
+    b->color = color;
+    std::vector<Affine> poses(nPoses);
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        const vm_batch_scene& d = scenes[i];
+        for (int f = 0; f < d.n_frames; ++f) {
+            const float* T = d.poses + 16 * (size_t)f;
+            float Ti[16];
+            if (!finite_all(T, 16) || !inverse4(T, Ti)) {
+                delete b;
+                return fail(VM_ERR_INVALID_ARG, "scene " + std::to_string(i) + ": pose " + std::to_string(f) +
+                                                    " is not finite or singular");
+            }
+            poses[off + f] = affine_of(T);
+            poses[off + d.n_frames + f] = affine_of(Ti);
+        }
+        off += 2 * (size_t)d.n_frames;
+    }
+    hipError_t e = hipMalloc((void**)&b->dPoses, sizeof(Affine) * nPoses);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dScenes, sizeof(BatchScene) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dFrame, sizeof(int) * 64);
+    if (e == hipSuccess) e = hipMemcpy(b->dPoses, poses.data(), sizeof(Affine) * nPoses, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->dFrame, 0, sizeof(int) * 64);
+    if (e != hipSuccess) {
+        vm_batch_destroy(b);
+        return hip_fail(e, "batch allocation");
+    }
+    off = 0;
+    for (int i = 0; i < n; ++i) {
+        const vm_batch_scene& d = scenes[i];
+        b->maps.push_back(d.map);
+        b->desc.push_back(d);
+        BatchScene S;
+        std::memset(&S, 0, sizeof(S));
+        S.m = d.map->view;
+        S.K = intrinsics_of(d.K);
+        S.d = DepthImage{(const uint8_t*)d.depth_frames, d.depth_pitch, d.width, d.height, df, d.depth_scale};
+        S.c = ColorImage{(const uint8_t*)d.rgb_frames, d.rgb_pitch};
+        S.depthFrameStride = d.depth_frame_stride;
+        S.rgbFrameStride = d.rgb_frame_stride;
+        S.poses = b->dPoses + off;
+        S.posesInv = b->dPoses + off + d.n_frames;
+        S.nFrames = d.n_frames;
+        S.vmap = d.vmap;
+        S.vpitch = d.vmap_pitch;
+        S.nmap = d.nmap;
+        S.npitch = d.nmap_pitch;
+        S.vsInv = (float)(1.0 / (double)S.m.voxelSize);
+        S.rtStep = S.m.trunc / S.m.voxelSize;
+        b->host.push_back(S);
+        off += 2 * (size_t)d.n_frames;
+    }
+    if (int rc = batch_upload(b)) {
+        vm_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return VM_OK;
+}
+
+int vm_batch_destroy(vm_batch* b) {
+    if (!b) return VM_OK;
+    batch_drop_graph(b);
+    if (b->dScenes) hipFree(b->dScenes);
+    if (b->dPoses) hipFree(b->dPoses);
+    if (b->dFrame) hipFree(b->dFrame);
+    delete b;
+    return VM_OK;
+}
+
+int vm_batch_set_frame(vm_batch* b, int32_t frame, void* stream) {
+    if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
+    if (frame < 0) return fail(VM_ERR_INVALID_ARG, "frame must be >= 0");
+    VM_HIP(batch_set_frame(b->dFrame, frame, (hipStream_t)stream), "set frame");
+    return VM_OK;
+}
+
+int vm_batch_get_frame(vm_batch* b, void* stream, int32_t* frame) {
+    if (!b || !frame) return fail(VM_ERR_INVALID_ARG, "batch and frame must not be NULL");
+    hipStream_t s = (hipStream_t)stream;
+    VM_HIP(hipMemcpyAsync(frame, b->dFrame, sizeof(int), hipMemcpyDeviceToHost, s), "frame copy");
+    VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    return VM_OK;
+}
+
+int vm_batch_step(vm_batch* b, void* stream) {
+    if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    if (b->exec) VM_HIP(hipGraphLaunch(b->exec, s), "graph launch");
+    else VM_HIP(batch_launch_step(b->view, b->tuning, b->fmt, b->df, b->color, s), "batch step");
+    batch_touch_maps(b);
+    return VM_OK;
+}
+
+int vm_batch_capture(vm_batch* b, void* stream) {
+    if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
+    if (!stream) return fail(VM_ERR_INVALID_ARG, "capture needs a non-null stream");
+    hipStream_t s = (hipStream_t)stream;
+    batch_drop_graph(b);
+    VM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipError_t e = batch_launch_step(b->view, b->tuning, b->fmt, b->df, b->color, s);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return hip_fail(e, "captured step");
+    }
+    if (e2 != hipSuccess) return hip_fail(e2, "end capture");
+    b->graph = g;
+    VM_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+    return VM_OK;
+}
+
+int vm_batch_get_tuning(vm_batch* b, vm_batch_tuning* out) {
+    if (!b || !out) return fail(VM_ERR_INVALID_ARG, "batch and out must not be NULL");
+    out->ray_tile_width = b->tuning.rayTileW;
+    out->xcd_bands = b->tuning.xcdBands;
+    out->visible_wg_per_cu = b->tuning.visibleWgPerCu;
+    out->integrate_wg_per_cu = b->tuning.integrateWgPerCu;
+    return VM_OK;
+}
+
+int vm_batch_set_tuning(vm_batch* b, const vm_batch_tuning* t) {
+    if (!b || !t) return fail(VM_ERR_INVALID_ARG, "batch and tuning must not be NULL");
+    if (t->ray_tile_width != 8 && t->ray_tile_width != 16 && t->ray_tile_width != 32)
+        return fail(VM_ERR_INVALID_ARG, "ray_tile_width must be 8, 16 or 32");
+    if (t->visible_wg_per_cu < 1 || t->visible_wg_per_cu > 64 || t->integrate_wg_per_cu < 1 ||
+        t->integrate_wg_per_cu > 64)
+        return fail(VM_ERR_INVALID_ARG, "workgroups per CU must be in [1, 64]");
+    batch_drop_graph(b);
+    b->tuning = BatchTuning{t->ray_tile_width, t->xcd_bands ? 1 : 0, t->visible_wg_per_cu, t->integrate_wg_per_cu};
+    return batch_upload(b);
+}
+
+// Autotuner: time every candidate launch shape over `steps` consecutive
+// steps (which fuse frames, like any step) and keep the fastest.
+int vm_batch_tune(vm_batch* b, void* stream, int32_t steps, vm_batch_tuning* best_out) {
+    if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
+    if (steps < 1) return fail(VM_ERR_INVALID_ARG, "steps must be >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    VM_HIP(hipEventCreate(&e0), "event");
+    VM_HIP(hipEventCreate(&e1), "event");
+    vm_batch_tuning best{8, 1, 4, 8};
+    float bestMs = 1e30f;
+    const int tws[] = {8, 16, 32};
+    const int xcds[] = {1, 0};
+    const int ints[] = {4, 8, 16};
+    int rc = VM_OK;
+    for (int tw : tws)
+        for (int xb : xcds)
+            for (int iw : ints) {
+                vm_batch_tuning t{tw, xb, 4, iw};
+                if ((rc = vm_batch_set_tuning(b, &t))) goto done;
+                if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(VM_ERR_HIP, "event record"); goto done; }
+                for (int k = 0; k < steps; ++k)
+                    if ((rc = vm_batch_step(b, stream))) goto done;
+                if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = fail(VM_ERR_HIP, "event timing");
+                    goto done;
+                }
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (ms < bestMs) { bestMs = ms; best = t; }
+            }
+    rc = vm_batch_set_tuning(b, &best);
+    if (rc == VM_OK && best_out) *best_out = best;
+done:
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
 }
 
 }  // extern "C"

@@ -82,6 +82,21 @@ struct Affine {
     float r20, r21, r22, t2;
 };
 
+enum DepthFormat : int { kDepthF32 = 0, kDepthU16 = 1 };
+
+struct DepthImage {
+    const uint8_t* data;
+    int64_t pitch;  // bytes between rows
+    int width, height;
+    int format;     // DepthFormat
+    float scale;    // metres per unit for kDepthU16
+};
+
+struct ColorImage {
+    const uint8_t* data;  // 3 bytes per pixel, or null
+    int64_t pitch;
+};
+
 // ---------------------------------------------------------------------------
 // exact helpers (host + device)
 

@@ -1,0 +1,213 @@
+// Device functions of the raycaster (MapRenderingDelegate,
+// ray_tracing.cu:255-401), shared by the single-map and multi-scene kernels.
+//
+// Every ray runs the reference's march step for step: the same sample
+// positions, the same reads, the same float operations in the same order.
+// Two things make it cheaper without changing any value:
+//   * each lane keeps the last two voxel blocks it resolved, so a sample that
+//     stays inside a block costs no hash probe;
+//   * inside a block that is not allocated every sample reads "invalid, 1.0"
+//     and steps exactly one voxel (res += 1.0f * dir == res + dir), so those
+//     iterations run as a tight loop of additions with a float-bounds test for
+//     leaving the block, and no reads at all.
+#pragma once
+
+#include "vm_common.h"
+
+namespace vmamd {
+
+// findEntry, map_struct.h:356-374 (read-only walk, bounded); returns ptr or -1
+__device__ inline int find_block(const MapView& m, int bx, int by, int bz) {
+    int slot = hash_block(bx, by, bz, m.nBucket, m.bucketMask);
+    int guard = m.nEntry - m.nBucket + 1;
+    while (true) {
+        const HashEntry* e = m.table + slot;
+        const int p = e->ptr, off = e->offset, x = e->x, y = e->y, z = e->z;
+        if (p >= 0 && x == bx && y == by && z == bz) return p;
+        if (off < 0 || --guard < 0) return -1;
+        slot = m.nBucket + off - 1;
+    }
+}
+
+template <int FMT>
+struct RayReader {
+    const MapView& m;
+    int cx, cy, cz, cp;  // most recent block
+    int px, py, pz, pp;  // the one before
+    __device__ explicit RayReader(const MapView& mv)
+        : m(mv), cx(INT_MIN), cy(INT_MIN), cz(INT_MIN), cp(-1), px(INT_MIN), py(INT_MIN), pz(INT_MIN), pp(-1) {}
+
+    __device__ inline int block(int bx, int by, int bz) {
+        if (bx == cx && by == cy && bz == cz) return cp;
+        if (bx == px && by == py && bz == pz) {
+            int t;
+            t = cx; cx = px; px = t;
+            t = cy; cy = py; py = t;
+            t = cz; cz = pz; pz = t;
+            t = cp; cp = pp; pp = t;
+            return cp;
+        }
+        const int p = find_block(m, bx, by, bz);
+        px = cx; py = cy; pz = cz; pp = cp;
+        cx = bx; cy = by; cz = bz; cp = p;
+        return p;
+    }
+
+    // voxel index of integer voxel (vx, vy, vz), -1 when its block is absent
+    __device__ inline int64_t voxel(int vx, int vy, int vz) {
+        const int p = block(vox_to_blk(vx), vox_to_blk(vy), vox_to_blk(vz));
+        if (p < 0) return -1;
+        return (int64_t)p + local_index(vox_to_loc(vx), vox_to_loc(vy), vox_to_loc(vz));
+    }
+
+    // read_sdf, ray_tracing.cu:272-286, for the voxel floor(q)
+    __device__ inline float read_at(int64_t i, bool& valid) {
+        float s = 1.0f;
+        bool v = false;
+        if (i >= 0) Vox<FMT>::read(m.voxels, i, s, v);
+        valid = v;
+        return v ? s : 1.0f;
+    }
+
+    // read_sdf_interped, ray_tracing.cu:288-321. floor(q + e_k) is computed
+    // per axis exactly as the reference adds the unit offset before flooring.
+    __device__ inline float interp(float3 q, bool& valid) {
+        const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+        const int X0 = floor_i(q.x), X1 = floor_i(q.x + 1.f);
+        const int Y0 = floor_i(q.y), Y1 = floor_i(q.y + 1.f);
+        const int Z0 = floor_i(q.z), Z1 = floor_i(q.z + 1.f);
+        int64_t idx[8];
+        idx[0] = voxel(X0, Y0, Z0);
+        idx[1] = voxel(X1, Y0, Z0);
+        idx[2] = voxel(X0, Y1, Z0);
+        idx[3] = voxel(X1, Y1, Z0);
+        idx[4] = voxel(X0, Y0, Z1);
+        idx[5] = voxel(X1, Y0, Z1);
+        idx[6] = voxel(X0, Y1, Z1);
+        idx[7] = voxel(X1, Y1, Z1);
+        float s[8];
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float v = 1.0f;
+            bool ok = false;
+            if (idx[k] >= 0) Vox<FMT>::read(m.voxels, idx[k], v, ok);
+            s[k] = ok ? v : 1.0f;
+            all = all && ok;
+        }
+        float r0 = (1.0f - fx) * s[0] + fx * s[1];
+        float r1 = (1.0f - fx) * s[2] + fx * s[3];
+        const float r2 = (1.0f - fy) * r0 + fy * r1;
+        r0 = (1.0f - fx) * s[4] + fx * s[5];
+        r1 = (1.0f - fx) * s[6] + fx * s[7];
+        const float r3 = (1.0f - fy) * r0 + fy * r1;
+        valid = all;
+        return (1.0f - fz) * r2 + fz * r3;
+    }
+};
+
+__device__ inline float norm3(float3 v) { return sqrtf(sum3(v.x * v.x, v.y * v.y, v.z * v.z)); }
+
+// |v| below 2^24: integer block bounds are exact floats there
+__device__ inline bool exact_range(int b) { return b > -(1 << 20) && b < (1 << 20); }
+
+// MapRenderingDelegate::operator() for pixel (x, y); out points at the
+// pixel's float4. vsInv = (float)(1.0 / voxelSize) (ray_tracing.cu:430),
+// rtStep = trunc / voxelSize (:431).
+template <int FMT>
+__device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y, const Intrinsics& K,
+                                     const Affine& pose, const Affine& Tinv, float vsInv, float rtStep) {
+    out[0] = __int_as_float(0x7fffffff);
+    const float zNear = 0.1f, zFar = 5.0f;  // MIN_DEPTH / MAX_DEPTH, ray_tracing.cu:340-341
+    const float ux = K.ifx * ((float)x - K.cx), uy = K.ify * ((float)y - K.cy);
+    float3 pt = make_float3(ux * zNear, uy * zNear, zNear);
+    float dist_s = norm3(pt) * vsInv;
+    float3 bs = xform(pose, pt);
+    bs = make_float3(bs.x * vsInv, bs.y * vsInv, bs.z * vsInv);
+    pt = make_float3(ux * zFar, uy * zFar, zFar);
+    const float distEnd = norm3(pt) * vsInv;
+    float3 be = xform(pose, pt);
+    be = make_float3(be.x * vsInv, be.y * vsInv, be.z * vsInv);
+    float3 dir = make_float3(be.x - bs.x, be.y - bs.y, be.z - bs.z);
+    {
+        // Eigen normalized(): divide by the norm when the squared norm is > 0
+        const float n2 = sum3(dir.x * dir.x, dir.y * dir.y, dir.z * dir.z);
+        if (n2 > 0.f) {
+            const float n = sqrtf(n2);
+            dir = make_float3(dir.x / n, dir.y / n, dir.z / n);
+        }
+    }
+    float3 res = bs;
+    RayReader<FMT> rd(m);
+    float sdf = 1.0f, last;
+    bool valid = false;
+    float step;
+    while (dist_s < distEnd) {
+        last = sdf;
+        const int vx = floor_i(res.x), vy = floor_i(res.y), vz = floor_i(res.z);
+        const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
+        const int p = rd.block(bx, by, bz);
+        if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
+            // unallocated block: read_sdf gives (1.0, invalid), step is 1
+            const float lx = (float)(bx * 8), ly = (float)(by * 8), lz = (float)(bz * 8);
+            const float hx = lx + 8.f, hy = ly + 8.f, hz = lz + 8.f;
+            do {
+                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+                dist_s += 1.0f;
+            } while (dist_s < distEnd && res.x >= lx && res.x < hx && res.y >= ly && res.y < hy && res.z >= lz &&
+                     res.z < hz);
+            sdf = 1.0f;
+            valid = false;
+            continue;
+        }
+        sdf = rd.read_at(p < 0 ? -1 : (int64_t)p + local_index(vox_to_loc(vx), vox_to_loc(vy), vox_to_loc(vz)),
+                         valid);
+        if (sdf <= 0.5f && sdf >= -0.5f) sdf = rd.interp(res, valid);
+        if (sdf <= 0.0f) break;
+        if (sdf >= 0.f && last < 0.f) return;
+        step = valid ? fmaxf(sdf * rtStep, 1.0f) : 1.0f;
+        res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
+        dist_s += step;
+    }
+    if (sdf <= 0.0f) {
+        step = sdf * rtStep;
+        res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
+        sdf = rd.interp(res, valid);
+        step = sdf * rtStep;
+        res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
+        if (valid) {
+            const float3 w = make_float3(res.x / vsInv, res.y / vsInv, res.z / vsInv);
+            const float3 c = xform(Tinv, w);
+            *(float4*)out = make_float4(c.x, c.y, c.z, 1.0f);
+        }
+    }
+}
+
+// computeNormalKernel body, image_proc.cu:74-93, for pixel (x, y) with
+// x < W - 1 and y < H - 1
+__device__ inline float3 normalized3(float3 a) {
+    const float n2 = sum3(a.x * a.x, a.y * a.y, a.z * a.z);
+    if (n2 > 0.f) {
+        const float n = sqrtf(n2);
+        return make_float3(a.x / n, a.y / n, a.z / n);
+    }
+    return a;
+}
+__device__ inline float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline void normal_pixel(const float* vmap, int64_t vpitch, float* nmap, int64_t npitch, int W, int H,
+                                    int x, int y) {
+    const int x10 = max(x - 1, 0), x01 = min(x + 1, W);
+    const int y10 = max(y - 1, 0), y01 = min(y + 1, H);
+    auto at = [&](int yy, int xx) {
+        const float4 v = *((const float4*)((const uint8_t*)vmap + (int64_t)yy * vpitch) + xx);
+        return make_float3(v.x, v.y, v.z);
+    };
+    const float3 v00 = at(y, x10), v01 = at(y, x01), v10 = at(y10, x), v11 = at(y01, x);
+    const float3 n = normalized3(cross3(make_float3(v01.x - v00.x, v01.y - v00.y, v01.z - v00.z),
+                                        make_float3(v11.x - v10.x, v11.y - v10.y, v11.z - v10.z)));
+    *((float4*)((uint8_t*)nmap + (int64_t)y * npitch) + x) = make_float4(n.x, n.y, n.z, 1.f);
+}
+
+}  // namespace vmamd

@@ -5,21 +5,6 @@
 
 namespace vmamd {
 
-enum DepthFormat : int { kDepthF32 = 0, kDepthU16 = 1 };
-
-struct DepthImage {
-    const uint8_t* data;
-    int64_t pitch;  // bytes between rows
-    int width, height;
-    int format;     // DepthFormat
-    float scale;    // metres per unit for kDepthU16
-};
-
-struct ColorImage {
-    const uint8_t* data;  // 3 bytes per pixel, or null
-    int64_t pitch;
-};
-
 // map_struct.cu:29-42 equivalent (plus the block-position index)
 hipError_t launch_reset(const MapView& m, hipStream_t s);
 // rebuild blockPos from the hash table (after state is loaded from outside)
@@ -30,6 +15,12 @@ hipError_t launch_allocate(const MapView& m, const DepthImage& d, const Intrinsi
 // CheckEntryVisibilityFunctor + DepthFusionFunctor/ImageFusionFunctor fused
 hipError_t launch_visible_fuse(const MapView& m, const DepthImage& d, const ColorImage& c,
                                const Intrinsics& K, const Affine& worldToCam, hipStream_t s);
+// the two halves of launch_visible_fuse: compaction of the visible blocks
+// (the count must be zero beforehand) and integration of the compacted list
+hipError_t launch_visible_compact(const MapView& m, int cols, int rows, const Intrinsics& K,
+                                  const Affine& worldToCam, hipStream_t s);
+hipError_t launch_integrate(const MapView& m, const DepthImage& d, const ColorImage& c, const Intrinsics& K,
+                            const Affine& worldToCam, hipStream_t s);
 // CheckEntryVisibilityFunctor alone (CheckVisibleBlocks, depth_fusion.cu:494-539)
 hipError_t launch_visible(const MapView& m, int width, int height, const Intrinsics& K,
                           const Affine& worldToCam, hipStream_t s);

@@ -11,6 +11,7 @@
 // order differs from the reference (which is already nondeterministic);
 // triangle contents, winding and normals follow :209-242 exactly.
 #include "vm_common.h"
+#include "vm_dev_ray.h"
 #include "vm_launch.h"
 #include "vm_mc_table.h"
 
@@ -19,18 +20,6 @@ namespace vmamd {
 namespace {
 
 __constant__ McTables c_mc = kMcTables;
-
-__device__ inline int find_block(const MapView& m, int bx, int by, int bz) {
-    int slot = hash_block(bx, by, bz, m.nBucket, m.bucketMask);
-    int guard = m.nEntry - m.nBucket + 1;
-    while (true) {
-        const HashEntry* e = m.table + slot;
-        const int p = e->ptr, off = e->offset, x = e->x, y = e->y, z = e->z;
-        if (p >= 0 && x == bx && y == by && z == bz) return p;
-        if (off < 0 || --guard < 0) return -1;
-        slot = m.nBucket + off - 1;
-    }
-}
 
 // interpolateLinear, marching_cubes.cu:106-115 (the 1e-6 guards compare in double)
 __device__ inline float interp_lin(float v1, float v2) {
@@ -57,18 +46,6 @@ __device__ inline float3 edge_vertex(int e, float3 p, const float* s) {
         case 10: t = interp_lin(s[2], s[6]); return make_float3(p.x + 1.f, p.y + 1.f, p.z + t);
         default: t = interp_lin(s[3], s[7]); return make_float3(p.x + 0.f, p.y + 1.f, p.z + t);
     }
-}
-
-__device__ inline float3 normalized3(float3 a) {
-    const float n2 = sum3(a.x * a.x, a.y * a.y, a.z * a.z);
-    if (n2 > 0.f) {
-        const float n = sqrtf(n2);
-        return make_float3(a.x / n, a.y / n, a.z / n);
-    }
-    return a;
-}
-__device__ inline float3 cross3(float3 a, float3 b) {
-    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
 constexpr int kHalo = 9 * 9 * 9;
@@ -185,16 +162,7 @@ __global__ void k_normals(const float* vmap, int64_t vpitch, float* nmap, int64_
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= W - 1 || y >= H - 1) return;
-    const int x10 = max(x - 1, 0), x01 = min(x + 1, W);
-    const int y10 = max(y - 1, 0), y01 = min(y + 1, H);
-    auto at = [&](int yy, int xx) {
-        const float4 v = *((const float4*)((const uint8_t*)vmap + (int64_t)yy * vpitch) + xx);
-        return make_float3(v.x, v.y, v.z);
-    };
-    const float3 v00 = at(y, x10), v01 = at(y, x01), v10 = at(y10, x), v11 = at(y01, x);
-    const float3 n = normalized3(cross3(make_float3(v01.x - v00.x, v01.y - v00.y, v01.z - v00.z),
-                                        make_float3(v11.x - v10.x, v11.y - v10.y, v11.z - v10.z)));
-    *((float4*)((uint8_t*)nmap + (int64_t)y * npitch) + x) = make_float4(n.x, n.y, n.z, 1.f);
+    normal_pixel(vmap, vpitch, nmap, npitch, W, H, x, y);
 }
 
 __device__ inline float saturate01(float v) { return v != v ? 0.f : fminf(1.f, fmaxf(0.f, v)); }

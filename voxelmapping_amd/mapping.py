@@ -265,6 +265,95 @@ def surface_points(verts, n_triangles: int, points=None, stream=None):
     return points[:n_triangles]
 
 
+class Batch:
+    """Many scenes fused and raycast in lockstep (vm_batch_* of the C ABI).
+
+    scenes: list of dicts with keys map (DeviceMap), K, depth (CUDA tensor
+    (F, H, W) of all frames, float32 metres or uint16), depth_scale, rgb
+    (CUDA uint8 (F, H, W, 3) or None), poses ((F, 4, 4) or (F, 16)
+    column-major camera-to-world). Output vertex/normal maps are allocated
+    here: ``vmaps[i]`` / ``nmaps[i]`` (H, W, 4) float32.
+    """
+
+    def __init__(self, scenes):
+        import torch
+        self._keep = []
+        descs = (_capi.BatchScene * len(scenes))()
+        self.vmaps, self.nmaps = [], []
+        for i, sc in enumerate(scenes):
+            dep = sc["depth"]
+            if dep.dim() != 3 or not dep.is_cuda:
+                raise ValueError("depth must be a CUDA tensor (frames, H, W)")
+            F, H, W = dep.shape
+            u16 = dep.dtype in (torch.uint16, torch.int16)
+            es = 2 if u16 else 4
+            vm = torch.zeros((H, W, 4), dtype=torch.float32, device=dep.device)
+            nm = torch.zeros_like(vm)
+            self.vmaps.append(vm)
+            self.nmaps.append(nm)
+            pa = np.asarray(sc["poses"], np.float32)
+            P = pa.transpose(0, 2, 1).reshape(F, 16) if pa.ndim == 3 else pa.reshape(F, 16)
+            P = np.ascontiguousarray(P)
+            self._keep.append(P)
+            d = descs[i]
+            d.map = sc["map"].handle
+            d.width, d.height = W, H
+            d.K[:] = list(intrinsics(sc["K"]))
+            d.depth_format = _capi.DEPTH_U16 if u16 else _capi.DEPTH_F32
+            d.depth_scale = float(sc.get("depth_scale", 1e-3))
+            d.depth_frames = dep.data_ptr()
+            d.depth_pitch = dep.stride(1) * es
+            d.depth_frame_stride = dep.stride(0) * es
+            rgb = sc.get("rgb")
+            if rgb is not None:
+                d.rgb_frames = rgb.data_ptr()
+                d.rgb_pitch = rgb.stride(1)
+                d.rgb_frame_stride = rgb.stride(0)
+            d.poses = P.ctypes.data
+            d.n_frames = F
+            d.vmap, d.vmap_pitch = vm.data_ptr(), W * 16
+            d.nmap, d.nmap_pitch = nm.data_ptr(), W * 16
+        self._scenes = scenes
+        h = C.c_void_p()
+        check(lib().vm_batch_create(descs, len(scenes), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vm_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_frame(self, f: int, stream=None):
+        check(lib().vm_batch_set_frame(self._h, int(f), _stream(stream)))
+
+    def frame(self, stream=None) -> int:
+        v = C.c_int32(0)
+        check(lib().vm_batch_get_frame(self._h, _stream(stream), C.byref(v)))
+        return int(v.value)
+
+    def step(self, stream=None):
+        check(lib().vm_batch_step(self._h, _stream(stream)))
+
+    def capture(self, stream=None):
+        check(lib().vm_batch_capture(self._h, _stream(stream)))
+
+    def tuning(self) -> dict:
+        t = _capi.BatchTuning()
+        check(lib().vm_batch_get_tuning(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _capi.BatchTuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = _capi.BatchTuning(**{**self.tuning(), **kw})
+        check(lib().vm_batch_set_tuning(self._h, C.byref(t)))
+
+    def tune(self, steps: int = 2, stream=None) -> dict:
+        t = _capi.BatchTuning()
+        check(lib().vm_batch_tune(self._h, _stream(stream), steps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _capi.BatchTuning._fields_}
+
+
 MAX_VERTS_BUFFER = 20_000_000  # triangles, vmapping.cpp:6
 
 
