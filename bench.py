@@ -179,6 +179,52 @@ def batched_bench(warmup: int, steps: int) -> dict:
     }
 
 
+def adjoint_bench(reps: int = 20) -> dict:
+    """Raycast forward vs its adjoint: 640x480 rays (fx = fy = 525) from a
+    pose perturbed by (1 cm, 0.5 deg) into the fused room_small volume (50
+    frames, 1 cm voxels, float TSDF), upstream gradients U[-1, 1] on every
+    pixel's depth and normal."""
+    import torch
+    from voxelmapping_amd import computeNormal, synth
+    from voxelmapping_amd.autograd import se3_exp
+    from voxelmapping_amd.mapping import DeviceMap
+    w = synth.room_small(voxel_format=2)
+    m = DeviceMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, 2)
+    for i in range(w.n_frames):
+        d, _ = w.frame(i, want_rgb=False)
+        m.fuse(torch.from_numpy(d).cuda(), w.K, w.poses(i + 1)[i])
+    rng = np.random.default_rng(3)
+    T = (w.poses()[-1].reshape(4, 4).T.astype(np.float64) @
+         se3_exp(np.r_[rng.normal(0, 0.01, 3), rng.normal(0, np.radians(0.5), 3)])).astype(np.float32)
+    K = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
+    H, W = 480, 640
+    vm = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    nm = torch.zeros_like(vm)
+    gv = torch.zeros_like(vm)
+    gv[..., 2] = torch.rand((H, W), device="cuda") * 2 - 1
+    gn = torch.rand((H, W, 4), device="cuda") * 2 - 1
+    gt = torch.zeros(w.n_block * 512, dtype=torch.float32, device="cuda")
+    gp = torch.zeros(6, dtype=torch.float64, device="cuda")
+    m.raycast(vm, K, T)
+    st = m.raycast_backward(vm, K, T, gt, gp, grad_vertex=gv, grad_normal=gn)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd = bwd = 0.0
+    for _ in range(reps):
+        ev[0].record()
+        m.raycast(vm, K, T)
+        computeNormal(vm, nm)
+        ev[1].record()
+        m.raycast_backward(vm, K, T, gt, gp, grad_vertex=gv, grad_normal=gn)
+        ev[2].record()
+        torch.cuda.synchronize()
+        fwd += ev[0].elapsed_time(ev[1])
+        bwd += ev[1].elapsed_time(ev[2])
+    return {"workload": "640x480 rays into room_small (1 cm, 50 frames), perturbed pose",
+            "forward_us": round(1000 * fwd / reps, 2), "backward_us": round(1000 * bwd / reps, 2),
+            "backward_over_forward": round(bwd / fwd, 3), "hit_pixels": st["hit_pixels"],
+            "overflow_pixels": st["overflow_pixels"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +343,7 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_bench(args.batched_warmup, args.batched_steps)
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth.room_vga(), args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -168,6 +168,28 @@ int vm_polygonize(vm_map* map, float* verts, float* norms, int64_t cap_triangles
 /* GetSurfacePoints (marching_cubes.cu:330-344): triangle centroids. */
 int vm_surface_points(const float* verts, float* points, int64_t n_triangles, void* stream);
 
+/* Adjoint of the raycast and the vertex-map normals (no counterpart in the
+ * reference): given the forward vertex map `vmap` of vm_raycast at this pose
+ * and upstream gradients d loss / d vertex (grad_vertex, float4 per pixel,
+ * xyz used) and d loss / d normal (grad_normal, float4 per pixel; either may
+ * be NULL), ACCUMULATE
+ *   grad_tsdf[i] += d loss / d sdf of voxel slot i (device, n_block * 512
+ *                   floats, indexed like voxel_block), and
+ *   grad_pose[k] += d loss / d xi_k (device, 6 doubles) for
+ *                   camToWorld = T exp(xi), xi = (rho_x, rho_y, rho_z, phi_x, phi_y, phi_z).
+ * The march is recomputed with its discrete decisions (cells, interpolation
+ * test, break step) held fixed; memory is O(pixels). Float voxel format only.
+ * Pixels whose march has more than 48 value-dependent steps contribute no
+ * gradient and are counted in overflow_pixels. */
+typedef struct vm_backward_stats {
+    int32_t hit_pixels;
+    int32_t overflow_pixels;
+} vm_backward_stats;
+int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, const float* grad_vertex,
+                        int64_t grad_vertex_pitch, const float* grad_normal, int64_t grad_normal_pitch, int32_t width,
+                        int32_t height, const float K[4], const float cam_to_world[16], float* grad_tsdf,
+                        double* grad_pose, void* stream, vm_backward_stats* out_stats);
+
 /* State transfer for tests and tools: copy a whole map array to/from HOST
  * memory. which: 0 hash table, 1 heap, 2 voxels, 3 counters (16 ints),
  * 4 visible table, 5 block positions. vm_map_upload rebuilds the block

@@ -44,6 +44,8 @@ def lib():
         L.or_surface_points.argtypes = [_P, _P, C.c_int64]
         L.or_project_rendering_blocks.restype = C.c_int
         L.or_project_rendering_blocks.argtypes = [_P, _P, _P, C.c_int, C.c_int, _P, _P]
+        L.or_raycast_pixels_f32.argtypes = [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P]
+        L.or_raycast_pixels_f64.argtypes = [_P, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int, _P, _P, _P]
         L.or_insert_block.restype = C.c_int
         L.or_insert_block.argtypes = [_P, C.c_int, C.c_int, C.c_int]
         L.or_map_counters.restype = C.c_int
@@ -140,6 +142,32 @@ class OracleMap:
         bc = np.zeros(1, np.int32)
         t = lib().or_polygonize(self.h, _p(v), _p(n), cap, _p(bc))
         return int(bc[0]), int(t), v[:t], n[:t]
+
+    def raycast_pixels_f32(self, px, py, K, T):
+        """(vertices n x 4, decision signatures, hit flags) of the float march."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        n = len(px)
+        v = np.zeros((n, 4), np.float32)
+        sig = np.zeros(n, np.uint64)
+        hit = np.zeros(n, np.int32)
+        lib().or_raycast_pixels_f32(self.h, n, _p(px), _p(py), _p(f32(K)), _p(f32(T)), _p(v), _p(sig), _p(hit))
+        return v, sig, hit.astype(bool)
+
+    def raycast_pixels_f64(self, px, py, K, T, pert_idx=None, pert_val=None):
+        """FP64 march with TSDF slot pert_idx[k] += pert_val[k]; T: 16 column-major doubles."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        n = len(px)
+        v = np.zeros((n, 4), np.float64)
+        sig = np.zeros(n, np.uint64)
+        hit = np.zeros(n, np.int32)
+        pi = np.ascontiguousarray(np.zeros(0) if pert_idx is None else pert_idx, np.int64)
+        pv = np.ascontiguousarray(np.zeros(0) if pert_val is None else pert_val, np.float64)
+        lib().or_raycast_pixels_f64(self.h, n, _p(px), _p(py), _p(np.ascontiguousarray(K, np.float64)),
+                                    _p(np.ascontiguousarray(T, np.float64).reshape(-1)), _p(pi), _p(pv), len(pi),
+                                    _p(v), _p(sig), _p(hit))
+        return v, sig, hit.astype(bool)
 
     def project_rendering_blocks(self, cols8, rows8, K, Twc):
         zx = np.zeros((rows8, cols8), np.float32)

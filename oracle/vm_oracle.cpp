@@ -524,6 +524,133 @@ void raycast(or_map* m, float* vmap, int W, int H, const float* K, const float* 
     }
 }
 
+/* ---------------------------------------------------------------------------
+ * The raycast march of ray_tracing.cu:323-400 templated on the scalar type,
+ * for finite-difference checks of the raycast adjoint: S = double evaluates
+ * the march in FP64 with TSDF values (sdf of the float map, plus sparse
+ * perturbations) and a double pose; S = float follows the float raycast
+ * above step for step. Both return a signature of the march's discrete
+ * decisions (iteration count, cells visited, interpolation flags, break) so a
+ * perturbation that changes the path can be detected. */
+template <typename S> struct V3T { S x, y, z; };
+
+struct Perturb {
+    const int64_t* idx;
+    const double* val;
+    int n;
+    double at(int64_t i) const {
+        for (int k = 0; k < n; ++k)
+            if (idx[k] == i) return val[k];
+        return 0.0;
+    }
+};
+
+template <typename S>
+struct MarchT {
+    or_map* m;
+    Perturb pert;
+    uint64_t sig;
+    void mix(uint64_t v) { sig = (sig ^ v) * 1099511628211ull; }
+    S read(V3T<S> p, bool* valid) {
+        I3 v{(int)std::floor(p.x), (int)std::floor(p.y), (int)std::floor(p.z)};
+        int e = find_entry(m, voxel_to_block(v));
+        if (e >= 0) {
+            int64_t idx = (int64_t)m->table[e].ptr + local_idx(voxel_to_local(v));
+            if (vox_valid(m, idx)) {
+                *valid = true;
+                return (S)vox_sdf(m, idx) + (S)pert.at(idx);
+            }
+        }
+        *valid = false;
+        return (S)1;
+    }
+    S interp(V3T<S> p, bool* valid) {
+        V3T<S> f{p.x - std::floor(p.x), p.y - std::floor(p.y), p.z - std::floor(p.z)};
+        bool vp, v;
+        S s0 = read(p, &vp), s1 = read(V3T<S>{p.x + 1, p.y, p.z}, &v); vp &= v;
+        S r0 = ((S)1 - f.x) * s0 + f.x * s1;
+        s0 = read(V3T<S>{p.x, p.y + 1, p.z}, &v); vp &= v;
+        s1 = read(V3T<S>{p.x + 1, p.y + 1, p.z}, &v); vp &= v;
+        S r1 = ((S)1 - f.x) * s0 + f.x * s1;
+        S r2 = ((S)1 - f.y) * r0 + f.y * r1;
+        s0 = read(V3T<S>{p.x, p.y, p.z + 1}, &v); vp &= v;
+        s1 = read(V3T<S>{p.x + 1, p.y, p.z + 1}, &v); vp &= v;
+        r0 = ((S)1 - f.x) * s0 + f.x * s1;
+        s0 = read(V3T<S>{p.x, p.y + 1, p.z + 1}, &v); vp &= v;
+        s1 = read(V3T<S>{p.x + 1, p.y + 1, p.z + 1}, &v); vp &= v;
+        r1 = ((S)1 - f.x) * s0 + f.x * s1;
+        S r3 = ((S)1 - f.y) * r0 + f.y * r1;
+        *valid = vp;
+        return ((S)1 - f.z) * r2 + f.z * r3;
+    }
+};
+
+template <typename S>
+inline V3T<S> xformT(const S* T, V3T<S> v) {  /* column-major 4x4, t + R v */
+    return V3T<S>{T[12] + (T[0] * v.x + (T[4] * v.y + T[8] * v.z)),
+                  T[13] + (T[1] * v.x + (T[5] * v.y + T[9] * v.z)),
+                  T[14] + (T[2] * v.x + (T[6] * v.y + T[10] * v.z))};
+}
+
+/* one pixel; returns hit, writes the camera-frame vertex */
+template <typename S>
+bool march_pixel(MarchT<S>& mt, int x, int y, const S* K, const S* T, const S* Ti, S vsInv, S rtStep, S* out) {
+    /* 1/f in double, rounded to S (the float form is depth_fusion.cu:209's) */
+    const S ifx = (S)(1.0 / (double)K[0]), ify = (S)(1.0 / (double)K[1]), cx = K[2], cy = K[3];
+    const S zNear = (S)0.1f, zFar = (S)5.f;
+    V3T<S> pt{ifx * ((S)x - cx) * zNear, ify * ((S)y - cy) * zNear, zNear};
+    S dist_s = std::sqrt(pt.x * pt.x + (pt.y * pt.y + pt.z * pt.z)) * vsInv;
+    V3T<S> bs = xformT(T, pt);
+    bs = V3T<S>{bs.x * vsInv, bs.y * vsInv, bs.z * vsInv};
+    pt = V3T<S>{ifx * ((S)x - cx) * zFar, ify * ((S)y - cy) * zFar, zFar};
+    const S distEnd = std::sqrt(pt.x * pt.x + (pt.y * pt.y + pt.z * pt.z)) * vsInv;
+    V3T<S> be = xformT(T, pt);
+    be = V3T<S>{be.x * vsInv, be.y * vsInv, be.z * vsInv};
+    V3T<S> dir{be.x - bs.x, be.y - bs.y, be.z - bs.z};
+    const S n2 = dir.x * dir.x + (dir.y * dir.y + dir.z * dir.z);
+    if (n2 > 0) {
+        const S n = std::sqrt(n2);
+        dir = V3T<S>{dir.x / n, dir.y / n, dir.z / n};
+    }
+    V3T<S> res = bs;
+    S sdf = 1, last, step;
+    bool valid = false;
+    int iters = 0;
+    while (dist_s < distEnd) {
+        last = sdf;
+        sdf = mt.read(res, &valid);
+        bool in = false;
+        if (sdf <= (S)0.5 && sdf >= (S)-0.5) {
+            sdf = mt.interp(res, &valid);
+            in = true;
+        }
+        ++iters;
+        if (valid)  /* value-carrying samples: their cells and branches */
+            mt.mix(((uint64_t)(uint32_t)(int)std::floor(res.x) << 32) ^ (uint64_t)(uint32_t)(int)std::floor(res.y) ^
+                   ((uint64_t)(uint32_t)(int)std::floor(res.z) << 16) ^ (in ? 7ull : 3ull) ^
+                   (sdf * rtStep > 1 ? 11ull : 0ull));
+        if (sdf <= 0) break;
+        if (sdf >= 0 && last < 0) { mt.mix(0xdeadull); return false; }
+        step = valid ? std::max(sdf * rtStep, (S)1) : (S)1;
+        res = V3T<S>{res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z};
+        dist_s += step;
+    }
+    mt.mix((uint64_t)iters);
+    if (!(sdf <= 0)) return false;
+    step = sdf * rtStep;
+    res = V3T<S>{res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z};
+    mt.mix(((uint64_t)(uint32_t)(int)std::floor(res.x) << 32) ^ (uint64_t)(uint32_t)(int)std::floor(res.y) ^
+           ((uint64_t)(uint32_t)(int)std::floor(res.z) << 16));
+    sdf = mt.interp(res, &valid);
+    step = sdf * rtStep;
+    res = V3T<S>{res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z};
+    if (!valid) { mt.mix(0xbeefull); return false; }
+    V3T<S> w{res.x / vsInv, res.y / vsInv, res.z / vsInv};
+    V3T<S> c = xformT(Ti, w);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = 1;
+    return true;
+}
+
 /* BuildVertexArray::interpolateLinear, marching_cubes.cu:106-115 */
 inline float interp_lin(float v1, float v2) {
     if ((double)std::fabs(0.0f - v1) < 1e-6) return 0;
@@ -784,6 +911,44 @@ int or_insert_block(or_map* m, int x, int y, int z) {
     create_block(m, I3{x, y, z});
     int e = find_entry(m, I3{x, y, z});
     return e >= 0 ? m->table[e].ptr : -1;
+}
+
+/* The raycast of a pixel list in FP32 (the float raycast's decisions, with
+ * their signature) or FP64 (TSDF values perturbed at pidx by pval). */
+void or_raycast_pixels_f32(or_map* m, int n, const int* px, const int* py, const float* K, const float* T,
+                           float* out_v, uint64_t* sig, int* hit) {
+    float Ti[16];
+    pose_inverse(T, Ti);
+    const float vsInv = (float)(1.0 / (double)m->voxelSize), rtStep = m->trunc / m->voxelSize;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(g_threads)
+    for (int i = 0; i < n; ++i) {
+        MarchT<float> mt{m, Perturb{nullptr, nullptr, 0}, 1469598103934665603ull};
+        float* o = out_v + (size_t)i * 4;
+        o[0] = NAN; o[1] = o[2] = o[3] = 0;
+        hit[i] = march_pixel<float>(mt, px[i], py[i], K, T, Ti, vsInv, rtStep, o);
+        sig[i] = mt.sig;
+    }
+}
+
+void or_raycast_pixels_f64(or_map* m, int n, const int* px, const int* py, const double* K, const double* T,
+                           const int64_t* pidx, const double* pval, int npert, double* out_v, uint64_t* sig,
+                           int* hit) {
+    /* rigid inverse in double: [R^T | -R^T t] */
+    double Ti[16] = {0};
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) Ti[c * 4 + r] = T[r * 4 + c];
+        Ti[12 + r] = -(T[r * 4 + 0] * T[12] + T[r * 4 + 1] * T[13] + T[r * 4 + 2] * T[14]);
+    }
+    Ti[15] = 1;
+    const double vsInv = 1.0 / (double)m->voxelSize, rtStep = (double)m->trunc / (double)m->voxelSize;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(g_threads)
+    for (int i = 0; i < n; ++i) {
+        MarchT<double> mt{m, Perturb{pidx, pval, npert}, 1469598103934665603ull};
+        double* o = out_v + (size_t)i * 4;
+        o[0] = NAN; o[1] = o[2] = o[3] = 0;
+        hit[i] = march_pixel<double>(mt, px[i], py[i], K, T, Ti, vsInv, rtStep, o);
+        sig[i] = mt.sig;
+    }
 }
 
 int or_map_counters(or_map* m, int* out) {

@@ -66,6 +66,17 @@ int or_project_rendering_blocks(or_map* m, float* zrange_x, float* zrange_y,
                                 int cols8, int rows8, const float* K,
                                 const float* T_world_to_cam);
 
+/* The raycast march for a list of pixels: FP32 (the float raycast's
+ * decisions) or FP64 with TSDF perturbations (voxel slot pidx[k] += pval[k]).
+ * out_v: n * 4 camera-frame vertices (x = NaN on a miss); sig: signature of
+ * the march's discrete decisions; hit: 1 on a hit. T (and K) are the same
+ * camera-to-world / intrinsics as or_raycast, in float or double. */
+void or_raycast_pixels_f32(or_map* m, int n, const int* px, const int* py, const float* K, const float* T,
+                           float* out_v, uint64_t* sig, int* hit);
+void or_raycast_pixels_f64(or_map* m, int n, const int* px, const int* py, const double* K, const double* T,
+                           const int64_t* pidx, const double* pval, int npert, double* out_v, uint64_t* sig,
+                           int* hit);
+
 /* createBlock (map_struct.h:304-354) for one key; returns its voxel ptr or -1 */
 int or_insert_block(or_map* m, int x, int y, int z);
 

@@ -451,3 +451,33 @@ def test_polygonize_sphere():
     assert abs(s.mean()) > 0.99
     # surface points are the centroids
     assert np.allclose(O.surface_points(v), cen, atol=1e-7)
+
+
+# --- FP32 / FP64 pixel marches used by the adjoint's finite-difference checks ----------
+
+def test_pixel_march_f32_equals_raycast():
+    m = plane_map(fmt=2, frames=2)
+    T = colmajor(np.eye(4))
+    full = m.raycast(W, H, KP, T, init=0.0)
+    ys, xs = np.mgrid[0:H, 0:W]
+    v, sig, hit = m.raycast_pixels_f32(xs.ravel(), ys.ravel(), KP, T)
+    assert np.array_equal(hit, ~np.isnan(full[..., 0]).ravel())
+    assert np.array_equal(v[hit], full.reshape(-1, 4)[hit])
+
+
+def test_pixel_march_f64_plane_and_pose_derivative():
+    # plane z = D in front of an identity camera: the centre ray's depth is D,
+    # and moving the camera along its own +z by h shortens it by h
+    from voxelmapping_amd.autograd import se3_exp
+    m = plane_map(fmt=2, frames=1)
+    px, py = np.array([15, 16]), np.array([11, 12])
+    v0, s0, h0 = m.raycast_pixels_f64(px, py, KP, np.eye(4).T.reshape(-1))
+    assert h0.all() and np.all(np.abs(v0[:, 2] - D) < 1e-4)
+    h = 1e-5
+    Tp = (np.eye(4) @ se3_exp([0, 0, h, 0, 0, 0])).T.reshape(-1)
+    Tm = (np.eye(4) @ se3_exp([0, 0, -h, 0, 0, 0])).T.reshape(-1)
+    vp, sp, _ = m.raycast_pixels_f64(px, py, KP, Tp)
+    vm, sm, _ = m.raycast_pixels_f64(px, py, KP, Tm)
+    assert np.array_equal(sp, s0) and np.array_equal(sm, s0)
+    dz = (vp[:, 2] - vm[:, 2]) / (2 * h)
+    assert np.allclose(dz, -1.0, atol=1e-3)

@@ -16,7 +16,7 @@ def test_frames_are_deterministic():
 
 
 def test_empty_room_depth_is_wall_distance():
-    w = synth.room_small(seed=1, n_spheres=0, n_boxes=0, noise=0.0, dropout=0.0)
+    w = synth.room_small(seed=1, n_spheres=0, n_boxes=0, noise=0.0, dropout=0.0, n_frames=1)
     # camera at the room centre looking along +x: the far wall is 2 m away
     T = np.eye(4, dtype=np.float32)
     T[:3, :3] = np.array([[0, 0, 1], [-1, 0, 0], [0, -1, 0]], np.float32).T.T
@@ -45,3 +45,12 @@ def test_random_walk_step_bounds():
         R = a[:3, :3].T @ b[:3, :3]
         ang = np.arccos(np.clip((np.trace(R) - 1) / 2, -1, 1))
         assert ang <= np.radians(1.0) + 1e-4
+
+
+def test_pose_i_independent_of_request_order():
+    a = synth.room_small(seed=2, n_frames=20)
+    b = synth.room_small(seed=2, n_frames=20)
+    for i in range(5):
+        a.frame(i)  # asks for poses(i + 1) one by one
+    assert np.array_equal(a.poses(20), b.poses(20))
+    assert np.array_equal(a.frame(3)[0], b.frame(3)[0])

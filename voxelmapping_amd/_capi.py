@@ -102,6 +102,10 @@ class BatchTuning(C.Structure):
     ]
 
 
+class BackwardStats(C.Structure):
+    _fields_ = [("hit_pixels", C.c_int32), ("overflow_pixels", C.c_int32)]
+
+
 _P = C.c_void_p
 _F4 = C.POINTER(C.c_float)
 _U32P = C.POINTER(C.c_uint32)
@@ -131,6 +135,8 @@ SIGNATURES = {
     "vm_render_shading": (C.c_int, [_P, C.c_int64, _P, C.c_int64, _P, C.c_int64, C.c_int32, C.c_int32, _P]),
     "vm_polygonize": (C.c_int, [_P, _P, _P, C.c_int64, _P, _U32P, _U32P]),
     "vm_surface_points": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "vm_raycast_backward": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, _P, C.c_int64, C.c_int32, C.c_int32, _F4,
+                                      _F4, _P, _P, _P, C.POINTER(BackwardStats)]),
     "vm_map_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "vm_map_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "vm_batch_create": (C.c_int, [C.POINTER(BatchScene), C.c_int32, C.POINTER(_P)]),

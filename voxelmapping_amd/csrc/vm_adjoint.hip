@@ -1,0 +1,391 @@
+// Adjoint of the raycaster: d(vertex map, normal map) / d(TSDF voxel values,
+// camera pose twist), for the float voxel format. No counterpart in the
+// reference; it differentiates its march (ray_tracing.cu:323-400) and its
+// normals (image_proc.cu:74-93) with the discrete decisions (cells, the
+// interpolate-or-not test, the break step) held fixed.
+//
+// Forward march, per pixel, in voxel units: r_0 = bs, r_{k+1} = r_k + s_k dir
+// with s_k = max(rtStep * sdf_k, 1) when the sample is valid, else 1; the
+// break step k* is the first with sdf <= 0; then r_a = r_k* + rtStep sdf_k* dir,
+// r_b = r_a + rtStep interp(r_a) dir, and the vertex is v = T^-1 (r_b / vsInv).
+// A step is "active" when its length depends on voxel values (valid and
+// rtStep * sdf > 1). Only active steps (and the break) carry gradient; the
+// others translate by exactly dir.
+//
+// Backward, per pixel: the march is recomputed with the same code path as
+// the forward kernel, active steps are pushed to a per-pixel stack in HBM
+// (position, type, and the count of unit steps after it), and the reverse
+// sweep applies lambda_k = lambda_{k+1} + (lambda_{k+1}.dir) rtStep grad s(r_k),
+// scattering (lambda.dir) rtStep w_j into dTSDF[j] with float atomics, and
+// accumulating d/d(dir), d/d(start) and the output transform into the pose
+// twist of camToWorld = T0 exp(xi), xi = (rho, phi). Memory is O(pixels).
+#include "vm_common.h"
+#include "vm_dev_ray.h"
+#include "vm_launch.h"
+
+namespace vmamd {
+
+namespace {
+
+constexpr int kStackDepth = 48;
+
+__device__ inline float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline float3 mul3(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ inline float dot3f(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline float3 crs3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// R * v and R^T * v of an Affine's rotation
+__device__ inline float3 rot(const Affine& T, float3 v) {
+    return make_float3(T.r00 * v.x + T.r01 * v.y + T.r02 * v.z, T.r10 * v.x + T.r11 * v.y + T.r12 * v.z,
+                       T.r20 * v.x + T.r21 * v.y + T.r22 * v.z);
+}
+__device__ inline float3 rotT(const Affine& T, float3 v) {
+    return make_float3(T.r00 * v.x + T.r10 * v.y + T.r20 * v.z, T.r01 * v.x + T.r11 * v.y + T.r21 * v.z,
+                       T.r02 * v.x + T.r12 * v.y + T.r22 * v.z);
+}
+
+// Trilinear sample at q with its voxel indices, weights and spatial
+// gradient (d sdf / d q), from the same corner set as RayReader::interp.
+struct Trilinear {
+    int64_t idx[8];
+    float w[8];
+    bool ok[8];
+    float value;
+    float3 grad;
+    bool valid;
+};
+
+__device__ inline void trilinear(RayReader<kFmtF32>& rd, float3 q, Trilinear& t) {
+    const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+    const int X0 = floor_i(q.x), X1 = floor_i(q.x + 1.f);
+    const int Y0 = floor_i(q.y), Y1 = floor_i(q.y + 1.f);
+    const int Z0 = floor_i(q.z), Z1 = floor_i(q.z + 1.f);
+    t.idx[0] = rd.voxel(X0, Y0, Z0);
+    t.idx[1] = rd.voxel(X1, Y0, Z0);
+    t.idx[2] = rd.voxel(X0, Y1, Z0);
+    t.idx[3] = rd.voxel(X1, Y1, Z0);
+    t.idx[4] = rd.voxel(X0, Y0, Z1);
+    t.idx[5] = rd.voxel(X1, Y0, Z1);
+    t.idx[6] = rd.voxel(X0, Y1, Z1);
+    t.idx[7] = rd.voxel(X1, Y1, Z1);
+    float s[8];
+    t.valid = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float v = 1.0f;
+        bool ok = false;
+        if (t.idx[k] >= 0) Vox<kFmtF32>::read(rd.m.voxels, t.idx[k], v, ok);
+        s[k] = ok ? v : 1.0f;
+        t.ok[k] = ok;
+        t.valid = t.valid && ok;
+    }
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    float r0 = gx * s[0] + fx * s[1];
+    float r1 = gx * s[2] + fx * s[3];
+    const float r2 = gy * r0 + fy * r1;
+    const float q0 = gx * s[4] + fx * s[5];
+    const float q1 = gx * s[6] + fx * s[7];
+    const float r3 = gy * q0 + fy * q1;
+    t.value = gz * r2 + fz * r3;
+    t.w[0] = gx * gy * gz; t.w[1] = fx * gy * gz; t.w[2] = gx * fy * gz; t.w[3] = fx * fy * gz;
+    t.w[4] = gx * gy * fz; t.w[5] = fx * gy * fz; t.w[6] = gx * fy * fz; t.w[7] = fx * fy * fz;
+    t.grad.x = gz * (gy * (s[1] - s[0]) + fy * (s[3] - s[2])) + fz * (gy * (s[5] - s[4]) + fy * (s[7] - s[6]));
+    t.grad.y = gz * (r1 - r0) + fz * (q1 - q0);
+    t.grad.z = r3 - r2;
+}
+
+// one reverse step: position q carried a sample s(q) used as rtStep * s(q) * dir
+// with multiplier lam; returns the added position adjoint, scatters dTSDF
+__device__ inline float3 reverse_sample(RayReader<kFmtF32>& rd, float3 q, bool interp, float3 lam, float3 dir,
+                                        float rtStep, float* gTsdf) {
+    const float a = dot3f(lam, dir) * rtStep;
+    if (interp) {
+        Trilinear t;
+        trilinear(rd, q, t);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (t.ok[k] && t.w[k] != 0.f) atomicAdd(gTsdf + t.idx[k], a * t.w[k]);
+        return mul3(t.grad, a);
+    }
+    const int64_t i = rd.voxel(floor_i(q.x), floor_i(q.y), floor_i(q.z));
+    if (i >= 0) atomicAdd(gTsdf + i, a);
+    return make_float3(0.f, 0.f, 0.f);
+}
+
+struct StackRec {
+    float x, y, z;
+    int meta;  // bit 0: interp sample; bits 1..: unit steps after this step
+};
+
+template <int TILE>
+__global__ __launch_bounds__(256) void k_raycast_backward(MapView m, const float* gv, int64_t gpitch, int W, int H,
+                                                          Intrinsics K, Affine pose, Affine Tinv, float vsInv,
+                                                          float rtStep, float* gTsdf, float* poseParts,
+                                                          StackRec* stack, int* stats) {
+    __shared__ float s_red[8][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tilesX = (W + 15) / 16;
+    const int x = (blockIdx.x % tilesX) * 16 + (threadIdx.x & 15);
+    const int y = (blockIdx.x / tilesX) * 16 + (threadIdx.x >> 4);
+    float part[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool overflow = false, hit = false;
+    if (x < W && y < H) {
+        const float4 g4 = *((const float4*)((const uint8_t*)gv + (int64_t)y * gpitch) + x);
+        const float3 gvec = make_float3(g4.x, g4.y, g4.z);
+        if (gvec.x != 0.f || gvec.y != 0.f || gvec.z != 0.f) {
+            const int64_t P = (int64_t)W * H, pix = (int64_t)y * W + x;
+            // ---- forward recompute (same decisions as raycast_pixel) ----
+            const float zNear = 0.1f, zFar = 5.0f;
+            const float ux = K.ifx * ((float)x - K.cx), uy = K.ify * ((float)y - K.cy);
+            const float3 pn = make_float3(ux * zNear, uy * zNear, zNear);
+            float dist_s = norm3(pn) * vsInv;
+            float3 bs = xform(pose, pn);
+            bs = make_float3(bs.x * vsInv, bs.y * vsInv, bs.z * vsInv);
+            const float3 pf = make_float3(ux * zFar, uy * zFar, zFar);
+            const float distEnd = norm3(pf) * vsInv;
+            float3 be = xform(pose, pf);
+            be = make_float3(be.x * vsInv, be.y * vsInv, be.z * vsInv);
+            const float3 dvec = make_float3(be.x - bs.x, be.y - bs.y, be.z - bs.z);
+            float3 dir = dvec;
+            float dn = 0.f;
+            {
+                const float n2 = sum3(dir.x * dir.x, dir.y * dir.y, dir.z * dir.z);
+                if (n2 > 0.f) {
+                    dn = sqrtf(n2);
+                    dir = make_float3(dir.x / dn, dir.y / dn, dir.z / dn);
+                }
+            }
+            float3 res = bs;
+            RayReader<kFmtF32> rd(m);
+            float sdf = 1.0f, last;
+            bool valid = false, interpUsed = false;
+            float step;
+            int depth = 0, lead = 0, units = 0;  // units: unit steps since the last pushed record
+            while (dist_s < distEnd) {
+                last = sdf;
+                const int vx = floor_i(res.x), vy = floor_i(res.y), vz = floor_i(res.z);
+                const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
+                const int p = rd.block(bx, by, bz);
+                if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
+                    const float lx = (float)(bx * 8), ly = (float)(by * 8), lz = (float)(bz * 8);
+                    const float hx = lx + 8.f, hy = ly + 8.f, hz = lz + 8.f;
+                    do {
+                        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+                        dist_s += 1.0f;
+                        ++units;
+                    } while (dist_s < distEnd && res.x >= lx && res.x < hx && res.y >= ly && res.y < hy &&
+                             res.z >= lz && res.z < hz);
+                    sdf = 1.0f;
+                    valid = false;
+                    continue;
+                }
+                sdf = rd.read_at(p < 0 ? -1 : (int64_t)p + local_index(vox_to_loc(vx), vox_to_loc(vy), vox_to_loc(vz)),
+                                 valid);
+                interpUsed = false;
+                if (sdf <= 0.5f && sdf >= -0.5f) {
+                    sdf = rd.interp(res, valid);
+                    interpUsed = true;
+                }
+                if (sdf <= 0.0f) break;
+                if (sdf >= 0.f && last < 0.f) break;  // the reference returns here: no hit
+                step = valid ? fmaxf(sdf * rtStep, 1.0f) : 1.0f;
+                const bool active = valid && sdf * rtStep > 1.0f;
+                if (active) {
+                    if (depth > 0) stack[(int64_t)(depth - 1) * P + pix].meta |= units << 1;
+                    else lead = units;
+                    if (depth < kStackDepth) {
+                        stack[(int64_t)depth * P + pix] = StackRec{res.x, res.y, res.z, interpUsed ? 1 : 0};
+                        ++depth;
+                    } else {
+                        overflow = true;
+                    }
+                    units = 0;
+                } else {
+                    ++units;
+                }
+                res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
+                dist_s += step;
+            }
+            if (sdf <= 0.0f && !overflow) {
+                const float3 rk = res;
+                const bool kInterp = interpUsed;
+                const float sk = sdf;
+                step = sdf * rtStep;
+                const float3 ra = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
+                Trilinear ta;
+                trilinear(rd, ra, ta);
+                const float sb = ta.value;
+                step = sb * rtStep;
+                const float3 rb = make_float3(ra.x + step * dir.x, ra.y + step * dir.y, ra.z + step * dir.z);
+                if (ta.valid) {
+                    hit = true;
+                    const float3 w = make_float3(rb.x / vsInv, rb.y / vsInv, rb.z / vsInv);
+                    const float3 u = xform(Tinv, w);  // camera-frame vertex
+                    // output transform v = R^T (w - t): d/dw = R g; twist part
+                    float3 lam = mul3(rot(pose, gvec), 1.0f / vsInv);
+                    float3 gRho = mul3(gvec, -1.f), gPhi = crs3(gvec, u);
+                    float3 gDir = make_float3(0.f, 0.f, 0.f);
+                    // r_b = r_a + rtStep s_b(r_a) dir
+                    {
+                        const float a = dot3f(lam, dir) * rtStep;
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if (ta.ok[k] && ta.w[k] != 0.f) atomicAdd(gTsdf + ta.idx[k], a * ta.w[k]);
+                        gDir = add3(gDir, mul3(lam, rtStep * sb));
+                        lam = add3(lam, mul3(ta.grad, a));
+                    }
+                    // r_a = r_k* + rtStep s_k* dir
+                    gDir = add3(gDir, mul3(lam, rtStep * sk));
+                    lam = add3(lam, reverse_sample(rd, rk, kInterp, lam, dir, rtStep, gTsdf));
+                    // unit steps before the break, then the active steps in reverse
+                    gDir = add3(gDir, mul3(lam, (float)units));
+                    for (int d = depth - 1; d >= 0; --d) {
+                        const StackRec rec = stack[(int64_t)d * P + pix];
+                        const float3 q = make_float3(rec.x, rec.y, rec.z);
+                        Trilinear t;
+                        float s;
+                        if (rec.meta & 1) {
+                            trilinear(rd, q, t);
+                            s = t.value;
+                        } else {
+                            float v = 1.f;
+                            bool ok = false;
+                            const int64_t i = rd.voxel(floor_i(q.x), floor_i(q.y), floor_i(q.z));
+                            if (i >= 0) Vox<kFmtF32>::read(m.voxels, i, v, ok);
+                            s = v;
+                        }
+                        gDir = add3(gDir, mul3(lam, s * rtStep + (float)(rec.meta >> 1)));
+                        lam = add3(lam, reverse_sample(rd, q, rec.meta & 1, lam, dir, rtStep, gTsdf));
+                    }
+                    gDir = add3(gDir, mul3(lam, (float)(depth > 0 ? lead : 0)));
+                    // dir = dvec / |dvec|; bs gets lam - d/d dvec, be gets d/d dvec
+                    float3 gD = make_float3(0.f, 0.f, 0.f);
+                    if (dn > 0.f) gD = mul3(sub3(gDir, mul3(dir, dot3f(dir, gDir))), 1.0f / dn);
+                    const float3 gBs = sub3(lam, gD), gBe = gD;
+                    // bs = vsInv (R p_n + t), be = vsInv (R p_f + t), T = T0 exp(xi)
+                    const float3 hBs = mul3(rotT(pose, gBs), vsInv), hBe = mul3(rotT(pose, gBe), vsInv);
+                    gRho = add3(gRho, add3(hBs, hBe));
+                    gPhi = add3(gPhi, add3(crs3(pn, hBs), crs3(pf, hBe)));
+                    part[0] = gRho.x; part[1] = gRho.y; part[2] = gRho.z;
+                    part[3] = gPhi.x; part[4] = gPhi.y; part[5] = gPhi.z;
+                }
+            }
+        }
+    }
+    // workgroup reduction of the pose parts into one slot per workgroup
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float v = part[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) s_red[wave][k] = v;
+    }
+    const unsigned long long hits = __ballot(hit), ovf = __ballot(overflow);
+    if (lane == 0) {
+        if (hits) atomicAdd(&stats[0], __popcll(hits));
+        if (ovf) atomicAdd(&stats[1], __popcll(ovf));
+    }
+    __syncthreads();
+    if (threadIdx.x < 6)
+        poseParts[(int64_t)blockIdx.x * 6 + threadIdx.x] =
+            s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] + s_red[3][threadIdx.x];
+}
+
+__global__ void k_pose_reduce(const float* parts, int n, double* gPose) {
+    __shared__ double s[256];
+    const int k = blockIdx.x;  // component
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += (double)parts[(int64_t)i * 6 + k];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gPose[k] += s[0];
+}
+
+// Adjoint of computeNormalKernel gathered per vertex: g_v[q] = g_vertex[q] +
+// the contributions of every normal pixel whose stencil reads q. Normals
+// computed from a missed vertex (x = NaN) pass no gradient.
+__global__ void k_normals_backward(const float* vmap, int64_t vpitch, const float* gvin, int64_t gvpitch,
+                                   const float* gn, int64_t gnpitch, float* gout, int64_t gopitch, int W, int H) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    auto V = [&](int yy, int xx) {
+        const float4 v = *((const float4*)((const uint8_t*)vmap + (int64_t)yy * vpitch) + xx);
+        return make_float3(v.x, v.y, v.z);
+    };
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    if (gvin) {
+        const float4 g = *((const float4*)((const uint8_t*)gvin + (int64_t)y * gvpitch) + x);
+        acc = make_float3(g.x, g.y, g.z);
+    }
+    if (gn) {
+        // normal pixels (px, py) whose stencil can read (x, y)
+        const int cand[5][2] = {{x, y}, {x - 1, y}, {x + 1, y}, {x, y - 1}, {x, y + 1}};
+        for (int c = 0; c < 5; ++c) {
+            const int px = cand[c][0], py = cand[c][1];
+            if (px < 0 || py < 0 || px >= W - 1 || py >= H - 1) continue;
+            const float4 g4 = *((const float4*)((const uint8_t*)gn + (int64_t)py * gnpitch) + px);
+            const float3 g = make_float3(g4.x, g4.y, g4.z);
+            if (g.x == 0.f && g.y == 0.f && g.z == 0.f) continue;
+            const int x10 = max(px - 1, 0), x01 = min(px + 1, W), y10 = max(py - 1, 0), y01 = min(py + 1, H);
+            const float3 v00 = V(py, x10), v01 = V(py, x01), v10 = V(y10, px), v11 = V(y01, px);
+            if (isnan(v00.x) || isnan(v01.x) || isnan(v10.x) || isnan(v11.x)) continue;
+            const float3 a = sub3(v01, v00), b = sub3(v11, v10);
+            const float3 cr = crs3(a, b);
+            const float n2 = sum3(cr.x * cr.x, cr.y * cr.y, cr.z * cr.z);
+            float3 gc = g;
+            if (n2 > 0.f) {
+                const float nn = sqrtf(n2);
+                const float3 nrm = mul3(cr, 1.0f / nn);
+                gc = mul3(sub3(g, mul3(nrm, dot3f(nrm, g))), 1.0f / nn);
+            }
+            const float3 ga = crs3(b, gc), gb = crs3(gc, a);
+            if (py == y && x01 == x) acc = add3(acc, ga);
+            if (py == y && x10 == x) acc = sub3(acc, ga);
+            if (px == x && y01 == y) acc = add3(acc, gb);
+            if (px == x && y10 == y) acc = sub3(acc, gb);
+        }
+    }
+    *((float4*)((uint8_t*)gout + (int64_t)y * gopitch) + x) = make_float4(acc.x, acc.y, acc.z, 0.f);
+}
+
+}  // namespace
+
+// scratch layout: [stack | pose parts (256-byte aligned) | vertex gradients]
+static size_t parts_bytes(int W, int H) {
+    const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
+    return ((sizeof(float) * 6 * tiles + 255) / 256) * 256;
+}
+
+size_t raycast_backward_scratch_bytes(int W, int H) {
+    return sizeof(StackRec) * kStackDepth * (size_t)W * H + parts_bytes(W, H) + sizeof(float) * 4 * (size_t)W * H;
+}
+
+hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t vpitch, const float* gVertex,
+                                   int64_t gvpitch, const float* gNormal, int64_t gnpitch, int W, int H,
+                                   const Intrinsics& K, const Affine& pose, const Affine& Tinv, float* gTsdf,
+                                   double* gPose, void* scratch, int* stats, hipStream_t s) {
+    const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
+    uint8_t* p = (uint8_t*)scratch;
+    StackRec* stack = (StackRec*)p;
+    p += sizeof(StackRec) * kStackDepth * (size_t)W * H;
+    float* parts = (float*)p;
+    p += parts_bytes(W, H);
+    float* gv = (float*)p;
+    const int64_t gpitch = (int64_t)W * 16;
+    hipLaunchKernelGGL(k_normals_backward, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, vmap, vpitch, gVertex,
+                       gvpitch, gNormal, gnpitch, gv, gpitch, W, H);
+    const float vsInv = (float)(1.0 / (double)m.voxelSize);
+    const float rtStep = m.trunc / m.voxelSize;
+    hipLaunchKernelGGL(k_raycast_backward<16>, dim3((unsigned)tiles), dim3(256), 0, s, m, gv, gpitch, W, H, K, pose,
+                       Tinv, vsInv, rtStep, gTsdf, parts, stack, stats);
+    hipLaunchKernelGGL(k_pose_reduce, dim3(6), dim3(256), 0, s, parts, (int)tiles, gPose);
+    return hipGetLastError();
+}
+
+}  // namespace vmamd

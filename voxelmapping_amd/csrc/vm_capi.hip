@@ -102,6 +102,9 @@ struct vm_map {
     float visT[16];
     // set by an allocation launch, which zeroes the visible count on the device
     bool visCountZeroed = false;
+    // scratch of the raycast adjoint, grown on demand
+    void* scratch = nullptr;
+    size_t scratchBytes = 0;
 };
 
 namespace {
@@ -223,7 +226,7 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
 int vm_map_destroy(vm_map* m) {
     if (!m) return VM_OK;
     MapView& v = m->view;
-    void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr};
+    void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr, m->scratch};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete m;
@@ -449,6 +452,48 @@ int vm_polygonize(vm_map* m, float* verts, float* norms, int64_t cap, void* stre
     }
     if (out_blocks) *out_blocks = (uint32_t)(m->view.nBlock - 1 - c[kHeapPtr]);
     if (out_tris) *out_tris = (uint32_t)(tris < cap ? tris : cap);
+    return VM_OK;
+}
+
+int vm_raycast_backward(vm_map* m, const float* vmap, int64_t vpitch, const float* gv, int64_t gvpitch,
+                        const float* gn, int64_t gnpitch, int32_t W, int32_t H, const float K[4], const float T[16],
+                        float* grad_tsdf, double* grad_pose, void* stream, vm_backward_stats* out_stats) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (m->view.fmt != VM_VOXEL_F32_F32_RGB8)
+        return fail(VM_ERR_INVALID_ARG, "the raycast adjoint needs the float voxel format");
+    if (int rc = check_image(vmap, vpitch, W, H, 16, "vmap")) return rc;
+    if (gv)
+        if (int rc = check_image(gv, gvpitch, W, H, 16, "grad_vertex")) return rc;
+    if (gn)
+        if (int rc = check_image(gn, gnpitch, W, H, 16, "grad_normal")) return rc;
+    if (vpitch % 16 || (gv && gvpitch % 16) || (gn && gnpitch % 16))
+        return fail(VM_ERR_INVALID_ARG, "map pitches must be multiples of 16");
+    if (!grad_tsdf || !grad_pose) return fail(VM_ERR_INVALID_ARG, "grad_tsdf and grad_pose must not be NULL");
+    if (int rc = check_pose_k(K, T)) return rc;
+    float Ti[16];
+    if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t need = raycast_backward_scratch_bytes(W, H) + 256;
+    if (need > m->scratchBytes) {
+        VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+        if (m->scratch) VM_HIP(hipFree(m->scratch), "scratch free");
+        m->scratch = nullptr;
+        m->scratchBytes = 0;
+        VM_HIP(hipMalloc(&m->scratch, need), "scratch allocation");
+        m->scratchBytes = need;
+    }
+    int* stats = (int*)((uint8_t*)m->scratch + need - 256);
+    VM_HIP(launch_fill_u32(stats, 0u, 2, s), "stats reset");
+    VM_HIP(launch_raycast_backward(m->view, vmap, vpitch, gv, gvpitch, gn, gnpitch, W, H, intrinsics_of(K),
+                                   affine_of(T), affine_of(Ti), grad_tsdf, grad_pose, m->scratch, stats, s),
+           "raycast backward");
+    if (out_stats) {
+        int st[2];
+        VM_HIP(hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, s), "stats copy");
+        VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+        out_stats->hit_pixels = st[0];
+        out_stats->overflow_pixels = st[1];
+    }
     return VM_OK;
 }
 

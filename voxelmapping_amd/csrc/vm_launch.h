@@ -45,6 +45,14 @@ hipError_t launch_polygonize(const MapView& m, float* verts, float* norms, int64
 // compute_surface_point_kernel, marching_cubes.cu:314-344
 hipError_t launch_surface_points(const float* verts, float* points, int64_t n, hipStream_t s);
 
+// adjoint of the raycast + normals (float voxel format); scratch must hold
+// raycast_backward_scratch_bytes(W, H); stats = {hit pixels, overflowed pixels}
+size_t raycast_backward_scratch_bytes(int W, int H);
+hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t vpitch, const float* gVertex,
+                                   int64_t gvpitch, const float* gNormal, int64_t gnpitch, int W, int H,
+                                   const Intrinsics& K, const Affine& camToWorld, const Affine& worldToCam,
+                                   float* gTsdf, double* gPose, void* scratch, int* stats, hipStream_t s);
+
 // device-side fill of 32-bit words (graph-capturable replacement of memset)
 hipError_t launch_fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t s);
 

@@ -89,10 +89,14 @@ class Workload:
         return self._scene
 
     def poses(self, n: int | None = None) -> np.ndarray:
+        """First n camera-to-world poses (column-major float[16] each). The
+        trajectory is generated for at least n_frames frames, so pose i does
+        not depend on how many poses were asked for."""
         n = self.n_frames if n is None else n
-        if self._poses is None or len(self._poses) < n:
-            out = np.zeros((n, 16), np.float32)
-            _synth_lib().vm_synth_trajectory(self.scene(), self.trajectory, n, self.seed * 7919 + 11,
+        total = max(n, self.n_frames)
+        if self._poses is None or len(self._poses) < total:
+            out = np.zeros((total, 16), np.float32)
+            _synth_lib().vm_synth_trajectory(self.scene(), self.trajectory, total, self.seed * 7919 + 11,
                                              self.max_step, self.max_rot, _ptr(out))
             self._poses = out
         return self._poses[:n]
