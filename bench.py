@@ -42,6 +42,21 @@ def stage_bytes(P, V):
     }
 
 
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` measured by the two
+    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known
+    size) recorded in profiles/<kernel>_<workload>_traffic.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip()), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def frame_bytes(P, V):
     return P * (B_DEPTH + B_COLOR + B_VMAP + B_NMAP) + V * (512 * (2 * S_VOXEL + S_RAY) + S_ENTRY)
 
@@ -323,9 +338,12 @@ def main():
     per_launch = stage_bytes(P, mean_vis)
     dom = max(names, key=lambda n: stage_avg_us[n])
     ach = per_launch[dom] / (stage_avg_us[dom] * 1e-6) / 1e9
+    traffic, traffic_src = profiled_traffic(dom, w.name)
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": int(per_launch[dom]), "avg_launch_us": round(stage_avg_us[dom], 2)}
+    if traffic is not None:
+        roofline["traffic_source"] = traffic_src
     out = {
         "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1000.0 * wall / args.steps, 4), "higher_is_better": True,
