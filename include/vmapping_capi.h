@@ -196,6 +196,12 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
  * index from the hash table after uploading the hash table. Synchronous. */
 int vm_map_download(vm_map* map, int32_t which, void* host_dst, int64_t bytes);
 int vm_map_upload(vm_map* map, int32_t which, const void* host_src, int64_t bytes);
+/* Voxels of blocks [first_block, first_block + n_blocks) to/from HOST memory
+ * (n_blocks * 512 * voxel bytes). No counterpart in the reference: with the
+ * whole-array calls above they save and load a map holding only its
+ * allocated blocks (ids [0, allocated_blocks)). Synchronous. */
+int vm_map_download_blocks(vm_map* map, int32_t first_block, int32_t n_blocks, void* host_dst);
+int vm_map_upload_blocks(vm_map* map, int32_t first_block, int32_t n_blocks, const void* host_src);
 
 /* ---------------------------------------------------------------------------
  * Multi-scene batches (no counterpart in the reference: its API fuses one

@@ -1,0 +1,79 @@
+"""TUM-format DatasetLoader (example/dataset_loader.cpp): parsing, comment
+skipping, nearest-timestamp pose association, quaternion convention (checked
+against scipy's Rotation), 16-bit depth decoding and the 1/5000 depth scale."""
+import numpy as np
+import pytest
+from PIL import Image
+
+from voxelmapping_amd.dataset import TUM_DEPTH_SCALE, DatasetLoader, quaternion_pose
+
+
+def write_sequence(root, depths, colors, frame_times, gt_rows, K=(525.0, 525.0, 319.5, 239.5)):
+    (root / "rgb").mkdir()
+    (root / "depth").mkdir()
+    lines = ["# color.txt and depth.txt associated", ""]
+    for i, (d, c, t) in enumerate(zip(depths, colors, frame_times)):
+        Image.fromarray(d.astype(np.uint16)).save(root / "depth" / f"{i}.png")
+        Image.fromarray(c).save(root / "rgb" / f"{i}.png")
+        lines.append(f"{t:.6f} rgb/{i}.png {t + 0.01:.6f} depth/{i}.png")
+    (root / "associated.txt").write_text("\n".join(lines) + "\n")
+    gt = ["# timestamp tx ty tz qx qy qz qw"] + [" ".join(f"{v:.9f}" for v in r) for r in gt_rows]
+    (root / "groundtruth.txt").write_text("\n".join(gt) + "\n")
+    (root / "calibration.txt").write_text(" ".join(str(v) for v in K) + "\n")
+
+
+def test_quaternion_pose_matches_scipy():
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        q = rng.normal(size=4) * rng.uniform(0.5, 3.0)  # unnormalised on purpose
+        t = rng.normal(size=3)
+        T = quaternion_pose(*t, *q)
+        R = Rotation.from_quat(q / np.linalg.norm(q)).as_matrix()  # scipy order (x, y, z, w)
+        assert np.allclose(T[:3, :3], R, atol=1e-12)
+        assert np.allclose(T[:3, 3], t) and np.allclose(T[3], [0, 0, 0, 1])
+    # 90 degrees about z maps x to y
+    s = np.sqrt(0.5)
+    T = quaternion_pose(0, 0, 0, 0, 0, s, s)
+    assert np.allclose(T[:3, :3] @ [1, 0, 0], [0, 1, 0])
+
+
+def test_loader_parses_and_associates(tmp_path):
+    rng = np.random.default_rng(1)
+    n = 4
+    depths = [rng.integers(0, 65535, (6, 8)).astype(np.uint16) for _ in range(n)]
+    colors = [rng.integers(0, 255, (6, 8, 3)).astype(np.uint8) for _ in range(n)]
+    times = [100.0, 100.5, 101.0, 101.5]
+    # ground truth at 10 Hz with a tie case: frame 101.5 is 0.05 s from both
+    # 101.45 and 101.55, the first of them wins
+    gt_times = [100.0 + 0.1 * k for k in range(15)] + [101.45, 101.55]
+    gt_rows = [(t, t, 2 * t, 0.1, 0.0, 0.0, np.sin(t), np.cos(t)) for t in gt_times]
+    write_sequence(tmp_path, depths, colors, times, gt_rows)
+
+    L = DatasetLoader(str(tmp_path))
+    assert L.loadImages(True) and len(L) == n
+    assert L.timeStamps == times
+    assert L.loadGroundTruth()
+    K = L.loadCalibration()
+    assert K.dtype == np.float32 and K[0, 0] == 525.0 and K[1, 2] == np.float32(239.5) and K[2, 2] == 1
+    for i in range(n):
+        d, c, t, T = L.GetNext()
+        assert d.dtype == np.uint16 and np.array_equal(d, depths[i])
+        assert c.dtype == np.uint8 and np.array_equal(c, colors[i])
+        assert t == times[i]
+        dist = np.abs(np.array(gt_times) - t)
+        best = int(np.flatnonzero(dist == dist.min())[0])
+        assert np.allclose(T, quaternion_pose(*gt_rows[best][1:]))
+    assert L.GetNext() is None
+    assert np.array_equal(L.getFirstFramePose(), L.groundTruth[0])
+    # depth scale of the example: 5000 units per metre
+    assert TUM_DEPTH_SCALE == 1.0 / 5000.0
+
+
+def test_loader_missing_files_raise(tmp_path):
+    L = DatasetLoader(str(tmp_path))
+    with pytest.raises(FileNotFoundError):
+        L.loadImages(True)
+    (tmp_path / "associated.txt").write_text("1.0 rgb/0.png\n")
+    with pytest.raises(ValueError):
+        L.loadImages(True)

@@ -139,6 +139,8 @@ SIGNATURES = {
                                       _F4, _P, _P, _P, C.POINTER(BackwardStats)]),
     "vm_map_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "vm_map_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "vm_map_download_blocks": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "vm_map_upload_blocks": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
     "vm_batch_create": (C.c_int, [C.POINTER(BatchScene), C.c_int32, C.POINTER(_P)]),
     "vm_batch_destroy": (C.c_int, [_P]),
     "vm_batch_set_frame": (C.c_int, [_P, C.c_int32, _P]),

@@ -188,6 +188,7 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
     v.depthMax = d->depth_max;
     v.maxWeight = m->desc.max_weight;
     v.fmt = d->voxel_format;
+    v.occMask = brick_words(v.nBlock) - 1;
     hipGetDevice(&m->device);
 
     const size_t vb = (size_t)voxel_bytes(v.fmt);
@@ -199,6 +200,7 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
         {(void**)&v.voxels, vb * kBlockVox * (size_t)v.nBlock, "voxel blocks"},
         {(void**)&v.blockPos, sizeof(int4) * (size_t)v.nBlock, "block positions"},
         {(void**)&v.ctr, sizeof(int) * kCounterInts, "counters"},
+        {(void**)&v.occ, sizeof(unsigned long long) * ((size_t)v.occMask + 1), "occupancy bricks"},
     };
     for (auto& a : allocs) {
         hipError_t e = hipMalloc(a.p, a.bytes);
@@ -226,7 +228,7 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
 int vm_map_destroy(vm_map* m) {
     if (!m) return VM_OK;
     MapView& v = m->view;
-    void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr, m->scratch};
+    void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr, v.occ, m->scratch};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete m;
@@ -562,11 +564,40 @@ int vm_map_upload(vm_map* m, int32_t which, const void* src, int64_t bytes) {
     VM_HIP(hipDeviceSynchronize(), "synchronize");
     VM_HIP(hipMemcpy(p, src, (size_t)n, hipMemcpyHostToDevice), "upload");
     if (which == 0) {
-        VM_HIP(launch_fill_u32(m->view.blockPos, (uint32_t)kEmptyKey, (int64_t)4 * m->view.nBlock, nullptr),
-               "block index clear");
         VM_HIP(launch_rebuild_index(m->view, nullptr), "block index rebuild");
         VM_HIP(hipStreamSynchronize(nullptr), "synchronize");
     }
+    m->epoch++;
+    m->visValid = false;
+    return VM_OK;
+}
+
+static int voxel_range(vm_map* m, int32_t first, int32_t count, const void* host, uint8_t** dev, size_t* bytes) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    const MapView& v = m->view;
+    if (!host || first < 0 || count < 0 || first > v.nBlock || count > v.nBlock - first)
+        return fail(VM_ERR_INVALID_ARG, "block range outside the voxel pool");
+    const size_t per = (size_t)voxel_bytes(v.fmt) * kBlockVox;
+    *dev = v.voxels + (size_t)first * per;
+    *bytes = (size_t)count * per;
+    return VM_OK;
+}
+
+int vm_map_download_blocks(vm_map* m, int32_t first_block, int32_t n_blocks, void* dst) {
+    uint8_t* p;
+    size_t n;
+    if (int rc = voxel_range(m, first_block, n_blocks, dst, &p, &n)) return rc;
+    VM_HIP(hipDeviceSynchronize(), "synchronize");
+    if (n) VM_HIP(hipMemcpy(dst, p, n, hipMemcpyDeviceToHost), "download");
+    return VM_OK;
+}
+
+int vm_map_upload_blocks(vm_map* m, int32_t first_block, int32_t n_blocks, const void* src) {
+    uint8_t* p;
+    size_t n;
+    if (int rc = voxel_range(m, first_block, n_blocks, src, &p, &n)) return rc;
+    VM_HIP(hipDeviceSynchronize(), "synchronize");
+    if (n) VM_HIP(hipMemcpy(p, src, n, hipMemcpyHostToDevice), "upload");
     m->epoch++;
     m->visValid = false;
     return VM_OK;

@@ -64,6 +64,8 @@ struct MapView {
     uint8_t* voxels;       // nBlock * 512 voxels of the map's format
     int4* blockPos;        // block id -> {x, y, z, hash slot}; x = kEmptyKey if dead
     int* ctr;              // Counter block
+    unsigned long long* occ;  // occupancy bricks, occMask + 1 words (see brick_slot)
+    uint32_t occMask;
     int nEntry, nBucket, nBlock;
     int bucketMask;        // nBucket - 1 when nBucket is a power of two, else -1
     float voxelSize, trunc, depthMin, depthMax, maxWeight;
@@ -110,13 +112,21 @@ __host__ __device__ inline float3 xform(const Affine& T, float3 v) {
     return o;
 }
 
-// float -> int conversion with the hardware's semantics made explicit:
-// NaN -> 0, out-of-range saturates (v_cvt_i32_f32 behaviour).
+// float -> int conversion toward zero, NaN -> 0, out-of-range saturating.
+// That is what v_cvt_i32_f32 computes, so the device uses the instruction
+// itself (a C++ cast of an out-of-range float would be undefined); the host
+// spells it out as selects over a clamped cast (2147483520 is the largest
+// float below 2^31).
 __host__ __device__ inline int f2i_sat(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return INT_MAX;
-    if (x < -2147483648.0f) return INT_MIN;
-    return (int)x;
+#if defined(__HIP_DEVICE_COMPILE__)
+    int r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+#else
+    const int r = (int)fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);
+    const int s = x >= 2147483648.0f ? INT_MAX : r;
+    return x != x ? 0 : s;
+#endif
 }
 __host__ __device__ inline int floor_i(float x) { return f2i_sat(floorf(x)); }
 
@@ -127,6 +137,26 @@ __host__ __device__ inline int hash_block(int x, int y, int z, int nBucket, int 
     if (mask >= 0) return (int)(r & (uint32_t)mask);
     int res = (int)r % nBucket;
     return res < 0 ? res + nBucket : res;
+}
+
+// Occupancy bricks: one 64-bit word per 4x4x4 voxel blocks, in a table of
+// occMask + 1 words indexed by a hash of the brick position. Words are not
+// keyed, so bricks may share one; a block's bit is set when the block is
+// published and never cleared until reset. A clear bit therefore proves that
+// no hash entry holds the block; a set bit is answered by the hash table.
+__host__ __device__ inline uint32_t brick_slot(int bx, int by, int bz, uint32_t mask) {
+    const uint32_t h = ((uint32_t)(bx >> 2) * 73856093u) ^ ((uint32_t)(by >> 2) * 19349669u) ^
+                       ((uint32_t)(bz >> 2) * 83492791u);
+    return h & mask;
+}
+__host__ __device__ inline unsigned long long brick_bit(int bx, int by, int bz) {
+    return 1ull << ((bx & 3) | ((by & 3) << 2) | ((bz & 3) << 4));
+}
+// words for a pool of nBlock blocks: a power of two in [2^12, 2^24]
+__host__ __device__ inline uint32_t brick_words(int nBlock) {
+    uint32_t w = 1u << 12;
+    while (w < (uint32_t)nBlock / 4u && w < (1u << 24)) w <<= 1;
+    return w;
 }
 
 // voxelPosToBlockPos (map_struct.h:131-141) is floor division by 8, i.e. an

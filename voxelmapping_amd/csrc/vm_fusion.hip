@@ -41,11 +41,19 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     for (int64_t j = (n4 << 2) + i; j < n; j += stride) p[j] = v;
 }
 
+__global__ void k_clear_block_pos(int4* blockPos, int nBlock) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nBlock) blockPos[i] = make_int4(kEmptyKey, kEmptyKey, kEmptyKey, -1);
+}
+
 __global__ void k_rebuild_index(MapView m) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.nEntry) return;
     HashEntry e = m.table[i];
-    if (e.ptr >= 0 && e.ptr / kBlockVox < m.nBlock) m.blockPos[e.ptr / kBlockVox] = make_int4(e.x, e.y, e.z, i);
+    if (e.ptr >= 0 && e.ptr / kBlockVox < m.nBlock) {
+        m.blockPos[e.ptr / kBlockVox] = make_int4(e.x, e.y, e.z, i);
+        atomicOr(m.occ + brick_slot(e.x, e.y, e.z, m.occMask), brick_bit(e.x, e.y, e.z));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -158,10 +166,16 @@ hipError_t launch_reset(const MapView& m, hipStream_t s) {
     int64_t bytes = (int64_t)m.nBlock * kBlockVox * voxel_bytes(m.fmt);
     hipError_t e = launch_fill_u32(m.voxels, 0u, bytes / 4, s);
     if (e != hipSuccess) return e;
+    e = launch_fill_u32(m.occ, 0u, 2 * ((int64_t)m.occMask + 1), s);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
+// block positions and occupancy bricks from the hash table alone
 hipError_t launch_rebuild_index(const MapView& m, hipStream_t s) {
+    hipError_t e = launch_fill_u32(m.occ, 0u, 2 * ((int64_t)m.occMask + 1), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_clear_block_pos, dim3((m.nBlock + 255) / 256), dim3(256), 0, s, m.blockPos, m.nBlock);
     hipLaunchKernelGGL(k_rebuild_index, dim3((m.nEntry + 255) / 256), dim3(256), 0, s, m);
     return hipGetLastError();
 }

@@ -83,7 +83,8 @@ class DeviceMap:
         d = _capi.MapDesc(n_entry, n_bucket, n_block, voxel_size, trunc_dist, depth_min, depth_max,
                           voxel_format, max_weight)
         h = C.c_void_p()
-        check(lib().vm_map_create(C.byref(d), C.byref(h)))
+        self._lib = lib()  # kept for release() during interpreter shutdown
+        check(self._lib.vm_map_create(C.byref(d), C.byref(h)))
         self._h = h
         self.desc = d
 
@@ -94,7 +95,7 @@ class DeviceMap:
 
     def release(self):
         if getattr(self, "_h", None):
-            lib().vm_map_destroy(self._h)
+            self._lib.vm_map_destroy(self._h)
             self._h = None
 
     __del__ = release
@@ -248,6 +249,48 @@ class DeviceMap:
 
     def counters(self) -> np.ndarray:
         return self.download(3).view(np.int32)
+
+    def download_blocks(self, first: int, count: int) -> np.ndarray:
+        out = np.empty(count * 512 * self.voxel_bytes, np.uint8)
+        check(lib().vm_map_download_blocks(self._h, first, count, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def upload_blocks(self, first: int, data: np.ndarray):
+        data = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+        per = 512 * self.voxel_bytes
+        if data.size % per:
+            raise ValueError(f"voxel data must be a whole number of {per}-byte blocks")
+        check(lib().vm_map_upload_blocks(self._h, first, data.size // per, data.ctypes.data_as(C.c_void_p)))
+
+    # map file: numpy .npz (no pickled objects) holding the descriptor, the
+    # hash table, the heap, the counters and the voxels of the allocated
+    # blocks [0, allocated); everything else of a map is its reset state
+    MAP_FILE_VERSION = 1
+
+    def save(self, path: str) -> None:
+        d = self.desc
+        counters = self.counters()
+        alloc = self.stats()["allocated_blocks"]
+        np.savez(path, version=np.int32(self.MAP_FILE_VERSION),
+                 desc_int=np.array([d.n_entry, d.n_bucket, d.n_block, d.voxel_format], np.int64),
+                 desc_float=np.array([d.voxel_size, d.trunc_dist, d.depth_min, d.depth_max, d.max_weight],
+                                     np.float32),
+                 hash_table=self.hash_table(), heap=self.download(1).view(np.int32), counters=counters,
+                 voxels=self.download_blocks(0, alloc))
+
+    @classmethod
+    def load(cls, path: str) -> "DeviceMap":
+        with np.load(path, allow_pickle=False) as z:
+            if int(z["version"]) != cls.MAP_FILE_VERSION:
+                raise ValueError(f"{path}: map file version {int(z['version'])} is not supported")
+            ne, nb, nv, fmt = (int(v) for v in z["desc_int"])
+            vs, tr, dmin, dmax, mw = (float(v) for v in z["desc_float"])
+            m = cls(ne, nb, nv, vs, tr, fmt, dmin, dmax, mw)
+            m.upload(0, z["hash_table"])
+            m.upload(1, z["heap"])
+            m.upload(3, z["counters"])
+            m.upload_blocks(0, z["voxels"])
+        return m
 
 
 def computeNormal(vmap, nmap=None, stream=None):
