@@ -190,6 +190,28 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
                         int32_t height, const float K[4], const float cam_to_world[16], float* grad_tsdf,
                         double* grad_pose, void* stream, vm_backward_stats* out_stats);
 
+/* Batched pose refinement (no counterpart in the reference): n_hyp
+ * camera-to-world hypotheses (HOST, column-major float[16] each) are refined
+ * against one depth image (device, formats as vm_fuse_depth) by `iters`
+ * Gauss-Newton iterations on the depth residual r = z_render(T) - depth over
+ * pixels (x, y) with x and y multiples of `stride`. z_render is the camera z
+ * of vm_raycast's vertex; its Jacobian by the twist xi of T exp(xi) is the
+ * exact derivative of that march (forward mode, the raycast adjoint's
+ * derivative). Residuals with |r| >= max_residual metres, misses and depth
+ * outside [depth_min, depth_max] are left out. Each iteration is one launch
+ * over all hypotheses; the 6x6 normal equations (Marquardt damping 1e-3) are
+ * solved in double and T <- T exp(delta). out_poses (HOST, n_hyp * 16,
+ * column-major) receives the refined poses; out_cost (HOST, n_hyp * (iters + 1)
+ * * 2 floats) the {mean r^2, residual count} before each update and after
+ * the last (mean -1 when no residual is left). out_system (HOST, may be NULL,
+ * n_hyp * (iters + 1) * 27 doubles) receives the normal equations of each
+ * evaluation: the 21 upper-triangle entries of J^T J row by row, then J^T r.
+ * Synchronous. */
+int vm_refine_poses(vm_map* map, const void* depth, int64_t depth_pitch, int32_t width, int32_t height,
+                    int32_t depth_format, float depth_scale, const float K[4], const float* init_poses, int32_t n_hyp,
+                    int32_t iters, int32_t stride, float max_residual, float* out_poses, float* out_cost,
+                    double* out_system, void* stream);
+
 /* State transfer for tests and tools: copy a whole map array to/from HOST
  * memory. which: 0 hash table, 1 heap, 2 voxels, 3 counters (16 ints),
  * 4 visible table, 5 block positions. vm_map_upload rebuilds the block

@@ -35,7 +35,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["vm_fusion.hip", "vm_raycast.hip", "vm_mesh.hip", "vm_batch.hip", "vm_adjoint.hip", "vm_capi.hip"]
+HIP_SOURCES = ["vm_fusion.hip", "vm_raycast.hip", "vm_mesh.hip", "vm_batch.hip", "vm_adjoint.hip", "vm_pose.hip",
+               "vm_capi.hip"]
 CXX_SOURCES = ["vm_facade.cpp"]
 
 LIB = os.path.join(PKG, "libvmapping_amd.so")

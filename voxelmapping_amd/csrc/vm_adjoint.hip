@@ -46,56 +46,6 @@ __device__ inline float3 rotT(const Affine& T, float3 v) {
                        T.r02 * v.x + T.r12 * v.y + T.r22 * v.z);
 }
 
-// Trilinear sample at q with its voxel indices, weights and spatial
-// gradient (d sdf / d q), from the same corner set as RayReader::interp.
-struct Trilinear {
-    int64_t idx[8];
-    float w[8];
-    bool ok[8];
-    float value;
-    float3 grad;
-    bool valid;
-};
-
-__device__ inline void trilinear(RayReader<kFmtF32>& rd, float3 q, Trilinear& t) {
-    const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
-    const int X0 = floor_i(q.x), X1 = floor_i(q.x + 1.f);
-    const int Y0 = floor_i(q.y), Y1 = floor_i(q.y + 1.f);
-    const int Z0 = floor_i(q.z), Z1 = floor_i(q.z + 1.f);
-    t.idx[0] = rd.voxel(X0, Y0, Z0);
-    t.idx[1] = rd.voxel(X1, Y0, Z0);
-    t.idx[2] = rd.voxel(X0, Y1, Z0);
-    t.idx[3] = rd.voxel(X1, Y1, Z0);
-    t.idx[4] = rd.voxel(X0, Y0, Z1);
-    t.idx[5] = rd.voxel(X1, Y0, Z1);
-    t.idx[6] = rd.voxel(X0, Y1, Z1);
-    t.idx[7] = rd.voxel(X1, Y1, Z1);
-    float s[8];
-    t.valid = true;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        float v = 1.0f;
-        bool ok = false;
-        if (t.idx[k] >= 0) Vox<kFmtF32>::read(rd.m.voxels, t.idx[k], v, ok);
-        s[k] = ok ? v : 1.0f;
-        t.ok[k] = ok;
-        t.valid = t.valid && ok;
-    }
-    const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
-    float r0 = gx * s[0] + fx * s[1];
-    float r1 = gx * s[2] + fx * s[3];
-    const float r2 = gy * r0 + fy * r1;
-    const float q0 = gx * s[4] + fx * s[5];
-    const float q1 = gx * s[6] + fx * s[7];
-    const float r3 = gy * q0 + fy * q1;
-    t.value = gz * r2 + fz * r3;
-    t.w[0] = gx * gy * gz; t.w[1] = fx * gy * gz; t.w[2] = gx * fy * gz; t.w[3] = fx * fy * gz;
-    t.w[4] = gx * gy * fz; t.w[5] = fx * gy * fz; t.w[6] = gx * fy * fz; t.w[7] = fx * fy * fz;
-    t.grad.x = gz * (gy * (s[1] - s[0]) + fy * (s[3] - s[2])) + fz * (gy * (s[5] - s[4]) + fy * (s[7] - s[6]));
-    t.grad.y = gz * (r1 - r0) + fz * (q1 - q0);
-    t.grad.z = r3 - r2;
-}
-
 // one reverse step: position q carried a sample s(q) used as rtStep * s(q) * dir
 // with multiplier lam; returns the added position adjoint, scatters dTSDF
 __device__ inline float3 reverse_sample(RayReader<kFmtF32>& rd, float3 q, bool interp, float3 lam, float3 dir,
@@ -157,6 +107,7 @@ __global__ __launch_bounds__(256) void k_raycast_backward(MapView m, const float
                     dir = make_float3(dir.x / dn, dir.y / dn, dir.z / dn);
                 }
             }
+            const float3 inv = make_float3(1.0f / fabsf(dir.x), 1.0f / fabsf(dir.y), 1.0f / fabsf(dir.z));
             float3 res = bs;
             RayReader<kFmtF32> rd(m);
             float sdf = 1.0f, last;
@@ -169,14 +120,7 @@ __global__ __launch_bounds__(256) void k_raycast_backward(MapView m, const float
                 const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
                 const int p = rd.block(bx, by, bz);
                 if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
-                    const float lx = (float)(bx * 8), ly = (float)(by * 8), lz = (float)(bz * 8);
-                    const float hx = lx + 8.f, hy = ly + 8.f, hz = lz + 8.f;
-                    do {
-                        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                        dist_s += 1.0f;
-                        ++units;
-                    } while (dist_s < distEnd && res.x >= lx && res.x < hx && res.y >= ly && res.y < hy &&
-                             res.z >= lz && res.z < hz);
+                    units += empty_walk(rd, res, dir, inv, dist_s, distEnd, bx, by, bz);
                     sdf = 1.0f;
                     valid = false;
                     continue;

@@ -499,6 +499,58 @@ int vm_raycast_backward(vm_map* m, const float* vmap, int64_t vpitch, const floa
     return VM_OK;
 }
 
+int vm_refine_poses(vm_map* m, const void* depth, int64_t pitch, int32_t W, int32_t H, int32_t fmt, float scale,
+                    const float K[4], const float* init_poses, int32_t n_hyp, int32_t iters, int32_t stride,
+                    float max_residual, float* out_poses, float* out_cost, double* out_system, void* stream) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (fmt != VM_DEPTH_F32 && fmt != VM_DEPTH_U16) return fail(VM_ERR_INVALID_ARG, "unknown depth_format");
+    if (int rc = check_image(depth, pitch, W, H, fmt == VM_DEPTH_F32 ? 4 : 2, "depth")) return rc;
+    if (n_hyp < 1 || n_hyp > 4096) return fail(VM_ERR_INVALID_ARG, "n_hyp must be in [1, 4096]");
+    if (iters < 0 || iters > 100) return fail(VM_ERR_INVALID_ARG, "iters must be in [0, 100]");
+    if (stride < 1) return fail(VM_ERR_INVALID_ARG, "stride must be >= 1");
+    if (!(max_residual > 0.f)) return fail(VM_ERR_INVALID_ARG, "max_residual must be > 0");
+    if (!init_poses || !out_poses || !out_cost)
+        return fail(VM_ERR_INVALID_ARG, "init_poses, out_poses and out_cost must not be NULL");
+    // column-major float -> row-major double; every hypothesis must be finite
+    std::vector<double> Tin((size_t)n_hyp * 16);
+    for (int h = 0; h < n_hyp; ++h) {
+        if (int rc = check_pose_k(K, init_poses + (size_t)h * 16)) return rc;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) Tin[(size_t)h * 16 + i * 4 + j] = init_poses[(size_t)h * 16 + j * 4 + i];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const size_t need = refine_poses_scratch_bytes(W, H, n_hyp, iters, stride);
+    if (need > m->scratchBytes) {
+        VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+        if (m->scratch) VM_HIP(hipFree(m->scratch), "scratch free");
+        m->scratch = nullptr;
+        m->scratchBytes = 0;
+        VM_HIP(hipMalloc(&m->scratch, need), "scratch allocation");
+        m->scratchBytes = need;
+    }
+    DepthImage d{(const uint8_t*)depth, pitch, W, H, fmt, scale};
+    double* posesD = nullptr;
+    float* cost = nullptr;
+    double* system = nullptr;
+    VM_HIP(launch_refine_poses(m->view, d, intrinsics_of(K), Tin.data(), n_hyp, iters, stride, max_residual, 1e-3,
+                               m->scratch, &posesD, &cost, &system, s),
+           "pose refinement");
+    if (out_system)
+        VM_HIP(hipMemcpyAsync(out_system, system, (size_t)n_hyp * (iters + 1) * 27 * sizeof(double),
+                              hipMemcpyDeviceToHost, s),
+               "system copy");
+    std::vector<double> Tout((size_t)n_hyp * 16);
+    VM_HIP(hipMemcpyAsync(Tout.data(), posesD, Tout.size() * sizeof(double), hipMemcpyDeviceToHost, s), "pose copy");
+    VM_HIP(hipMemcpyAsync(out_cost, cost, (size_t)n_hyp * (iters + 1) * 2 * sizeof(float), hipMemcpyDeviceToHost, s),
+           "cost copy");
+    VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    for (int h = 0; h < n_hyp; ++h)
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                out_poses[(size_t)h * 16 + j * 4 + i] = (float)Tout[(size_t)h * 16 + i * 4 + j];
+    return VM_OK;
+}
+
 int vm_surface_points(const float* verts, float* points, int64_t n, void* stream) {
     if (n < 0) return fail(VM_ERR_INVALID_ARG, "n_triangles must be >= 0");
     if (n > 0 && (!verts || !points)) return fail(VM_ERR_INVALID_ARG, "verts and points must not be NULL");

@@ -158,6 +158,10 @@ __host__ __device__ inline uint32_t brick_words(int nBlock) {
     while (w < (uint32_t)nBlock / 4u && w < (1u << 24)) w <<= 1;
     return w;
 }
+// set a published block's occupancy bit
+__device__ inline void occupancy_mark(const MapView& m, int x, int y, int z) {
+    atomicOr(m.occ + brick_slot(x, y, z, m.occMask), brick_bit(x, y, z));
+}
 
 // voxelPosToBlockPos (map_struct.h:131-141) is floor division by 8, i.e. an
 // arithmetic shift; voxelPosToLocalPos (:148-162) is the Euclidean remainder.

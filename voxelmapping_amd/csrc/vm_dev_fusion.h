@@ -59,7 +59,7 @@ __device__ inline void publish(const MapView& m, int slot, int x, int y, int z, 
     st(&e->y, y);
     st(&e->z, z);
     m.blockPos[blk] = make_int4(x, y, z, slot);
-    atomicOr(m.occ + brick_slot(x, y, z, m.occMask), brick_bit(x, y, z));
+    occupancy_mark(m, x, y, z);
     __hip_atomic_store(&e->ptr, blk * kBlockVox, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 

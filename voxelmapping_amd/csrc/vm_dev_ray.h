@@ -20,12 +20,14 @@ namespace vmamd {
 
 // The excess chain behind a bucket head that did not hold the key: the tail of
 // findEntry, map_struct.h:356-374 (read-only walk, bounded); ptr or -1.
+// (The key tests use & rather than && so that the five fields of an entry are
+// loaded together instead of the position only after ptr has arrived.)
 __device__ inline int chain_walk(const MapView& m, int off, int bx, int by, int bz) {
     int guard = m.nEntry - m.nBucket + 1;
     while (off >= 0 && --guard >= 0) {
         const HashEntry* e = m.table + (m.nBucket + off - 1);
         const int p = e->ptr, x = e->x, y = e->y, z = e->z;
-        if (p >= 0 && x == bx && y == by && z == bz) return p;
+        if ((p >= 0) & (x == bx) & (y == by) & (z == bz)) return p;
         off = e->offset;
     }
     return -1;
@@ -35,7 +37,7 @@ __device__ inline int chain_walk(const MapView& m, int off, int bx, int by, int 
 __device__ inline int find_block(const MapView& m, int bx, int by, int bz) {
     const HashEntry* e = m.table + hash_block(bx, by, bz, m.nBucket, m.bucketMask);
     const int p = e->ptr, off = e->offset, x = e->x, y = e->y, z = e->z;
-    if (p >= 0 && x == bx && y == by && z == bz) return p;
+    if ((p >= 0) & (x == bx) & (y == by) & (z == bz)) return p;
     return chain_walk(m, off, bx, by, bz);
 }
 
@@ -129,14 +131,29 @@ struct RayReader {
             s[k] = ok ? v : 1.0f;
             all = all && ok;
         }
+        valid = all;
+        return blend(fx, fy, fz, s);
+    }
+
+    // the trilinear blend of read_sdf_interped, ray_tracing.cu:305-320
+    __device__ static inline float blend(float fx, float fy, float fz, const float s[8]) {
         float r0 = (1.0f - fx) * s[0] + fx * s[1];
         float r1 = (1.0f - fx) * s[2] + fx * s[3];
         const float r2 = (1.0f - fy) * r0 + fy * r1;
         r0 = (1.0f - fx) * s[4] + fx * s[5];
         r1 = (1.0f - fx) * s[6] + fx * s[7];
         const float r3 = (1.0f - fy) * r0 + fy * r1;
-        valid = all;
         return (1.0f - fz) * r2 + fz * r3;
+    }
+
+    // The march's sample at q, floor(q) = (vx, vy, vz) in block p (ptr or
+    // -1): read_sdf of floor(q), and read_sdf_interped of q when that value
+    // is within [-0.5, 0.5] (ray_tracing.cu:358-362).
+    __device__ inline float sample(float3 q, int vx, int vy, int vz, int p, bool& valid) {
+        const int lx = vox_to_loc(vx), ly = vox_to_loc(vy), lz = vox_to_loc(vz);
+        float sdf = read_at(p < 0 ? -1 : (int64_t)p + local_index(lx, ly, lz), valid);
+        if (sdf <= 0.5f && sdf >= -0.5f) sdf = interp(q, valid);
+        return sdf;
     }
 };
 
@@ -145,21 +162,135 @@ __device__ inline float norm3(float3 v) { return sqrtf(sum3(v.x * v.x, v.y * v.y
 // |v| below 2^24: integer block bounds are exact floats there
 __device__ inline bool exact_range(int b) { return b > -(1 << 20) && b < (1 << 20); }
 
-// MapRenderingDelegate::operator() for pixel (x, y); out points at the
-// pixel's float4. vsInv = (float)(1.0 / voxelSize) (ray_tracing.cu:430),
-// rtStep = trunc / voxelSize (:431).
+// Trilinear sample at q with its voxel indices, weights and spatial
+// gradient (d sdf / d q): the same corners and the same operations as
+// RayReader::interp, so `value` and `valid` equal what interp returns.
+struct Trilinear {
+    int64_t idx[8];
+    float w[8];
+    bool ok[8];
+    float value;
+    float3 grad;
+    bool valid;
+};
+
 template <int FMT>
-__device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y, const Intrinsics& K,
-                                     const Affine& pose, const Affine& Tinv, float vsInv, float rtStep) {
-    out[0] = __int_as_float(0x7fffffff);
+__device__ inline void trilinear(RayReader<FMT>& rd, float3 q, Trilinear& t) {
+    const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+    const int X0 = floor_i(q.x), X1 = floor_i(q.x + 1.f);
+    const int Y0 = floor_i(q.y), Y1 = floor_i(q.y + 1.f);
+    const int Z0 = floor_i(q.z), Z1 = floor_i(q.z + 1.f);
+    t.idx[0] = rd.voxel(X0, Y0, Z0);
+    t.idx[1] = rd.voxel(X1, Y0, Z0);
+    t.idx[2] = rd.voxel(X0, Y1, Z0);
+    t.idx[3] = rd.voxel(X1, Y1, Z0);
+    t.idx[4] = rd.voxel(X0, Y0, Z1);
+    t.idx[5] = rd.voxel(X1, Y0, Z1);
+    t.idx[6] = rd.voxel(X0, Y1, Z1);
+    t.idx[7] = rd.voxel(X1, Y1, Z1);
+    float s[8];
+    t.valid = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float v = 1.0f;
+        bool ok = false;
+        if (t.idx[k] >= 0) Vox<FMT>::read(rd.m.voxels, t.idx[k], v, ok);
+        s[k] = ok ? v : 1.0f;
+        t.ok[k] = ok;
+        t.valid = t.valid && ok;
+    }
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    float r0 = gx * s[0] + fx * s[1];
+    float r1 = gx * s[2] + fx * s[3];
+    const float r2 = gy * r0 + fy * r1;
+    const float q0 = gx * s[4] + fx * s[5];
+    const float q1 = gx * s[6] + fx * s[7];
+    const float r3 = gy * q0 + fy * q1;
+    t.value = gz * r2 + fz * r3;
+    t.w[0] = gx * gy * gz; t.w[1] = fx * gy * gz; t.w[2] = gx * fy * gz; t.w[3] = fx * fy * gz;
+    t.w[4] = gx * gy * fz; t.w[5] = fx * gy * fz; t.w[6] = gx * fy * fz; t.w[7] = fx * fy * fz;
+    t.grad.x = gz * (gy * (s[1] - s[0]) + fy * (s[3] - s[2])) + fz * (gy * (s[5] - s[4]) + fy * (s[7] - s[6]));
+    t.grad.y = gz * (r1 - r0) + fz * (q1 - q0);
+    t.grad.z = r3 - r2;
+}
+
+// The march's samples in blocks that no hash entry holds (read_sdf gives
+// (1.0, invalid), the step is one voxel), starting at res in such a block
+// (bx, by, bz) whose bounds are exact floats. Runs the same additions as the
+// reference's loop and stops after the first sample outside the blocks known
+// to be empty, or at distEnd. Returns the number of unit steps taken.
+// inv = 1 / |dir| per axis.
+template <int FMT>
+__device__ inline int empty_walk(RayReader<FMT>& rd, float3& res, const float3 dir, const float3 inv, float& dist_s,
+                                 const float distEnd, const int bx, const int by, const int bz) {
+    // The walk goes on across a face into the next block when that block's
+    // bit in the brick word the lane holds is clear.
+    int cbx = bx, cby = by, cbz = bz, steps = 0;
+    while (true) {
+        const float lx = (float)(cbx * 8), ly = (float)(cby * 8), lz = (float)(cbz * 8);
+        const float hx = lx + 8.f, hy = ly + 8.f, hz = lz + 8.f;
+        // Unit steps that stay strictly inside the block and below distEnd
+        // whatever the rounding of the additions: k <= 64 additions to
+        // coordinates below B move them at most 64 * 2^-24 * B from the exact
+        // k * dir, which the margins cover. They are the same additions as
+        // the checked steps, without the checks.
+        {
+            const float ex = (fabsf(lx) + 9.f) * 0x1p-18f + 0x1p-16f;
+            const float ey = (fabsf(ly) + 9.f) * 0x1p-18f + 0x1p-16f;
+            const float ez = (fabsf(lz) + 9.f) * 0x1p-18f + 0x1p-16f;
+            const float kx = ((dir.x > 0.f ? hx - res.x : res.x - lx) - ex) * inv.x;
+            const float ky = ((dir.y > 0.f ? hy - res.y : res.y - ly) - ey) * inv.y;
+            const float kz = ((dir.z > 0.f ? hz - res.z : res.z - lz) - ez) * inv.z;
+            // minimum that keeps a NaN (no unchecked steps then)
+            float q = fminf((distEnd - dist_s) - 2.0f, 64.0f);
+            q = kx >= q ? q : kx;
+            q = ky >= q ? q : ky;
+            q = kz >= q ? q : kz;
+            int n = q >= 1.0f ? (int)q : 0;
+            steps += n + 1;
+            for (; n >= 2; n -= 2) {
+                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+                dist_s += 1.0f;
+                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+                dist_s += 1.0f;
+            }
+            if (n) {
+                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+                dist_s += 1.0f;
+            }
+        }
+        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+        dist_s += 1.0f;
+        if (!(dist_s < distEnd)) break;
+        if (res.x >= lx && res.x < hx && res.y >= ly && res.y < hy && res.z >= lz && res.z < hz) continue;
+        cbx = vox_to_blk(floor_i(res.x));
+        cby = vox_to_blk(floor_i(res.y));
+        cbz = vox_to_blk(floor_i(res.z));
+        if (!(exact_range(cbx) && exact_range(cby) && exact_range(cbz)) || !rd.known_absent(cbx, cby, cbz))
+            break;
+    }
+    return steps;
+}
+
+// State of one ray of MapRenderingDelegate::operator() (ray_tracing.cu:330-401)
+// between samples, so that a lane can run a ray in pieces.
+struct Ray {
+    float3 res, dir, inv;  // inv = 1 / |dir| per axis (+inf for a zero component)
+    float dist_s, distEnd, sdf;
+    bool valid;
+};
+
+// Ray setup for pixel (x, y). vsInv = (float)(1.0 / voxelSize)
+// (ray_tracing.cu:430).
+__device__ inline void ray_begin(Ray& r, int x, int y, const Intrinsics& K, const Affine& pose, float vsInv) {
     const float zNear = 0.1f, zFar = 5.0f;  // MIN_DEPTH / MAX_DEPTH, ray_tracing.cu:340-341
     const float ux = K.ifx * ((float)x - K.cx), uy = K.ify * ((float)y - K.cy);
     float3 pt = make_float3(ux * zNear, uy * zNear, zNear);
-    float dist_s = norm3(pt) * vsInv;
+    r.dist_s = norm3(pt) * vsInv;
     float3 bs = xform(pose, pt);
     bs = make_float3(bs.x * vsInv, bs.y * vsInv, bs.z * vsInv);
     pt = make_float3(ux * zFar, uy * zFar, zFar);
-    const float distEnd = norm3(pt) * vsInv;
+    r.distEnd = norm3(pt) * vsInv;
     float3 be = xform(pose, pt);
     be = make_float3(be.x * vsInv, be.y * vsInv, be.z * vsInv);
     float3 dir = make_float3(be.x - bs.x, be.y - bs.y, be.z - bs.z);
@@ -171,91 +302,77 @@ __device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y,
             dir = make_float3(dir.x / n, dir.y / n, dir.z / n);
         }
     }
-    // 1 / |dir| per axis for the step counts of the empty-block walk
-    // (+inf for a zero component: that axis never bounds the walk)
-    const float3 inv = make_float3(1.0f / fabsf(dir.x), 1.0f / fabsf(dir.y), 1.0f / fabsf(dir.z));
-    float3 res = bs;
+    r.dir = dir;
+    r.inv = make_float3(1.0f / fabsf(dir.x), 1.0f / fabsf(dir.y), 1.0f / fabsf(dir.z));
+    r.res = bs;
+    r.sdf = 1.0f;
+    r.valid = false;
+}
+
+enum RayStatus : int { kRayGoing = 0, kRaySurface = 1, kRayMiss = 2 };
+
+// One pass of the march loop body (ray_tracing.cu:355-379): a sample, or a
+// run of samples in empty blocks. kRaySurface: the loop broke on sdf <= 0;
+// kRayMiss: it returned on a sign change or ran out of distance.
+// rtStep = trunc / voxelSize (ray_tracing.cu:431).
+template <int FMT>
+__device__ inline int ray_step(Ray& r, RayReader<FMT>& rd, float rtStep) {
+    if (!(r.dist_s < r.distEnd)) return kRayMiss;  // loop condition; sdf > 0 or NaN here
+    const float last = r.sdf;
+    const int vx = floor_i(r.res.x), vy = floor_i(r.res.y), vz = floor_i(r.res.z);
+    const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
+    const int p = rd.block(bx, by, bz);
+    if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
+        empty_walk(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, bx, by, bz);
+        r.sdf = 1.0f;
+        r.valid = false;
+        return kRayGoing;
+    }
+    bool valid;
+    const float sdf = rd.sample(r.res, vx, vy, vz, p, valid);
+    r.sdf = sdf;
+    r.valid = valid;
+    if (sdf <= 0.0f) return kRaySurface;
+    if (sdf >= 0.f && last < 0.f) return kRayMiss;
+    const float step = valid ? fmaxf(sdf * rtStep, 1.0f) : 1.0f;
+    r.res = make_float3(r.res.x + step * r.dir.x, r.res.y + step * r.dir.y, r.res.z + step * r.dir.z);
+    r.dist_s += step;
+    return kRayGoing;
+}
+
+// The two zero-crossing corrections and the output (ray_tracing.cu:381-399)
+// after kRaySurface: the camera-frame vertex, or false when the last
+// interpolation is invalid.
+template <int FMT>
+__device__ inline bool ray_surface(Ray& r, RayReader<FMT>& rd, const Affine& Tinv, float vsInv, float rtStep,
+                                   float4& v) {
+    float step = r.sdf * rtStep;
+    float3 res = make_float3(r.res.x + step * r.dir.x, r.res.y + step * r.dir.y, r.res.z + step * r.dir.z);
+    bool valid;
+    const float sdf = rd.interp(res, valid);
+    step = sdf * rtStep;
+    res = make_float3(res.x + step * r.dir.x, res.y + step * r.dir.y, res.z + step * r.dir.z);
+    if (!valid) return false;
+    const float3 w = make_float3(res.x / vsInv, res.y / vsInv, res.z / vsInv);
+    const float3 c = xform(Tinv, w);
+    v = make_float4(c.x, c.y, c.z, 1.0f);
+    return true;
+}
+
+// MapRenderingDelegate::operator() for pixel (x, y); out points at the
+// pixel's float4. A miss writes only x = NaN, as the reference does.
+template <int FMT>
+__device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y, const Intrinsics& K,
+                                     const Affine& pose, const Affine& Tinv, float vsInv, float rtStep) {
+    Ray r;
+    ray_begin(r, x, y, K, pose, vsInv);
     RayReader<FMT> rd(m);
-    float sdf = 1.0f, last;
-    bool valid = false;
-    float step;
-    while (dist_s < distEnd) {
-        last = sdf;
-        const int vx = floor_i(res.x), vy = floor_i(res.y), vz = floor_i(res.z);
-        const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
-        const int p = rd.block(bx, by, bz);
-        if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
-            // unallocated block: read_sdf gives (1.0, invalid), step is 1.
-            // The walk goes on across a face into the next block when that
-            // block's bit in the brick word the lane holds is clear.
-            int cbx = bx, cby = by, cbz = bz;
-            while (true) {
-                const float lx = (float)(cbx * 8), ly = (float)(cby * 8), lz = (float)(cbz * 8);
-                const float hx = lx + 8.f, hy = ly + 8.f, hz = lz + 8.f;
-                // unit steps that stay strictly inside the block and below
-                // distEnd whatever the rounding of the additions: k <= 64
-                // additions to coordinates below B move them at most
-                // 64 * 2^-24 * B from the exact k * dir, which the margins
-                // cover. The steps are the same additions, unchecked.
-                {
-                    const float ex = (fabsf(lx) + 9.f) * 0x1p-18f + 0x1p-16f;
-                    const float ey = (fabsf(ly) + 9.f) * 0x1p-18f + 0x1p-16f;
-                    const float ez = (fabsf(lz) + 9.f) * 0x1p-18f + 0x1p-16f;
-                    const float kx = ((dir.x > 0.f ? hx - res.x : res.x - lx) - ex) * inv.x;
-                    const float ky = ((dir.y > 0.f ? hy - res.y : res.y - ly) - ey) * inv.y;
-                    const float kz = ((dir.z > 0.f ? hz - res.z : res.z - lz) - ez) * inv.z;
-                    // minimum that keeps a NaN (no unchecked steps then)
-                    float q = fminf((distEnd - dist_s) - 2.0f, 64.0f);
-                    q = kx >= q ? q : kx;
-                    q = ky >= q ? q : ky;
-                    q = kz >= q ? q : kz;
-                    int n = q >= 1.0f ? (int)q : 0;
-                    for (; n >= 2; n -= 2) {
-                        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                        dist_s += 1.0f;
-                        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                        dist_s += 1.0f;
-                    }
-                    if (n) {
-                        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                        dist_s += 1.0f;
-                    }
-                }
-                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                dist_s += 1.0f;
-                if (!(dist_s < distEnd)) break;
-                if (res.x >= lx && res.x < hx && res.y >= ly && res.y < hy && res.z >= lz && res.z < hz) continue;
-                cbx = vox_to_blk(floor_i(res.x));
-                cby = vox_to_blk(floor_i(res.y));
-                cbz = vox_to_blk(floor_i(res.z));
-                if (!(exact_range(cbx) && exact_range(cby) && exact_range(cbz)) || !rd.known_absent(cbx, cby, cbz))
-                    break;
-            }
-            sdf = 1.0f;
-            valid = false;
-            continue;
-        }
-        sdf = rd.read_at(p < 0 ? -1 : (int64_t)p + local_index(vox_to_loc(vx), vox_to_loc(vy), vox_to_loc(vz)),
-                         valid);
-        if (sdf <= 0.5f && sdf >= -0.5f) sdf = rd.interp(res, valid);
-        if (sdf <= 0.0f) break;
-        if (sdf >= 0.f && last < 0.f) return;
-        step = valid ? fmaxf(sdf * rtStep, 1.0f) : 1.0f;
-        res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
-        dist_s += step;
+    int st;
+    while ((st = ray_step(r, rd, rtStep)) == kRayGoing) {
     }
-    if (sdf <= 0.0f) {
-        step = sdf * rtStep;
-        res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
-        sdf = rd.interp(res, valid);
-        step = sdf * rtStep;
-        res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
-        if (valid) {
-            const float3 w = make_float3(res.x / vsInv, res.y / vsInv, res.z / vsInv);
-            const float3 c = xform(Tinv, w);
-            *(float4*)out = make_float4(c.x, c.y, c.z, 1.0f);
-        }
-    }
+    float4 v;
+    if (st == kRaySurface && ray_surface(r, rd, Tinv, vsInv, rtStep, v)) *(float4*)out = v;
+    else out[0] = __int_as_float(0x7fffffff);
 }
 
 // computeNormalKernel body, image_proc.cu:74-93, for pixel (x, y) with

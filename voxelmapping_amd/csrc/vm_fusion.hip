@@ -52,7 +52,7 @@ __global__ void k_rebuild_index(MapView m) {
     HashEntry e = m.table[i];
     if (e.ptr >= 0 && e.ptr / kBlockVox < m.nBlock) {
         m.blockPos[e.ptr / kBlockVox] = make_int4(e.x, e.y, e.z, i);
-        atomicOr(m.occ + brick_slot(e.x, e.y, e.z, m.occMask), brick_bit(e.x, e.y, e.z));
+        occupancy_mark(m, e.x, e.y, e.z);
     }
 }
 

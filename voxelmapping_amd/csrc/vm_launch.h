@@ -58,4 +58,10 @@ hipError_t launch_fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t
 
 int num_compute_units();
 
+// batched Gauss-Newton pose refinement (vm_pose.hip)
+size_t refine_poses_scratch_bytes(int width, int height, int nHyp, int iters, int stride);
+hipError_t launch_refine_poses(const MapView& m, const DepthImage& d, const Intrinsics& K, const double* posesIn,
+                               int nHyp, int iters, int stride, float maxRes, double lambda, void* scratch,
+                               double** posesOut, float** costOut, double** systemOut, hipStream_t s);
+
 }  // namespace vmamd

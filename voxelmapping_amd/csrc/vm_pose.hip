@@ -1,0 +1,358 @@
+// Batched pose refinement: Gauss-Newton on the depth residual
+// z(T) - depth over a pixel grid, for many camera-pose hypotheses in one
+// launch per iteration. No counterpart in the reference; it builds on its
+// raycast (MapRenderingDelegate, ray_tracing.cu:323-400).
+//
+// z(T) of a pixel is the depth of the map's zero level along the pixel's ray:
+// the raycast's hit (the same march and corrections) refined by Newton steps
+// on the trilinear TSDF, p <- p - s(p) / (grad s . dir) dir. For camToWorld =
+// T exp(xi), xi = (rho, phi), the hit p = o + t dir (o the ray start) keeps
+// s(p) = 0, so dt = -grad s . (do + t ddir) / (grad s . dir) and
+// dp = do + t ddir + dt dir (implicit differentiation); the camera point is
+// v = exp(-xi) T^-1 p, dv = R^T dp - rho - phi x v. The raycast's own hit is
+// not used as z: its two fixed corrections stop short of the zero level on
+// oblique rays by a pose-dependent amount, which makes its exact derivative
+// (the raycast adjoint's) a poor Gauss-Newton model.
+//
+// Each iteration accumulates J^T J, J^T r, sum r^2 and the residual count per
+// hypothesis (per-workgroup partials, reduced in double), solves the damped
+// 6x6 normal equations by Cholesky in double and right-multiplies the pose by
+// exp(delta).
+#include <algorithm>
+
+#include "vm_common.h"
+#include "vm_dev_fusion.h"
+#include "vm_dev_ray.h"
+#include "vm_launch.h"
+
+namespace vmamd {
+
+namespace {
+
+constexpr int kGnTerms = 29;  // 21 of J^T J (upper triangle), 6 of J^T r, sum r^2, count
+constexpr int kGnThreads = 256;
+
+__device__ inline float3 f3add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline float3 f3mul(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ inline float f3dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline float3 rmul(const Affine& T, float3 v) {
+    return make_float3(T.r00 * v.x + T.r01 * v.y + T.r02 * v.z, T.r10 * v.x + T.r11 * v.y + T.r12 * v.z,
+                       T.r20 * v.x + T.r21 * v.y + T.r22 * v.z);
+}
+
+// d(T exp(xi) p)/d xi_j for a camera-frame point p: R e_j (j < 3), R (e_j x p)
+__device__ inline float3 point_tangent(const Affine& T, float3 p, int j) {
+    float3 e = make_float3(0.f, 0.f, 0.f);
+    if (j == 0) e.x = 1.f;
+    else if (j == 1) e.y = 1.f;
+    else if (j == 2) e.z = 1.f;
+    else if (j == 3) e = make_float3(0.f, -p.z, p.y);
+    else if (j == 4) e = make_float3(p.z, 0.f, -p.x);
+    else e = make_float3(-p.y, p.x, 0.f);
+    return rmul(T, e);
+}
+
+// Newton steps from the raycast's hit onto the zero level of the trilinear
+// TSDF along the ray
+constexpr int kZeroSteps = 3;
+
+// Depth of pixel (x, y) at pose T and its derivative by the pose twist.
+// Returns false when the raycast misses or the zero level is not found.
+template <int FMT>
+__device__ inline bool surface_depth(RayReader<FMT>& rd, int x, int y, const Intrinsics& K, const Affine& T,
+                                     const Affine& Tinv, float vsInv, float rtStep, float& z, float J[6]) {
+    Ray r;
+    ray_begin(r, x, y, K, T, vsInv);
+    const float3 o = r.res;
+    int st;
+    while ((st = ray_step(r, rd, rtStep)) == kRayGoing) {
+    }
+    if (st != kRaySurface) return false;
+    float step = r.sdf * rtStep;
+    float3 p = make_float3(r.res.x + step * r.dir.x, r.res.y + step * r.dir.y, r.res.z + step * r.dir.z);
+    Trilinear t;
+    trilinear(rd, p, t);
+    if (!t.valid) return false;
+    step = t.value * rtStep;
+    p = make_float3(p.x + step * r.dir.x, p.y + step * r.dir.y, p.z + step * r.dir.z);
+    // zero level along the ray
+    for (int k = 0; k < kZeroSteps; ++k) {
+        trilinear(rd, p, t);
+        const float den = f3dot(t.grad, r.dir);
+        if (!t.valid || !(fabsf(den) > 1e-6f)) return false;
+        p = f3add(p, f3mul(r.dir, -t.value / den));
+    }
+    trilinear(rd, p, t);
+    const float den = f3dot(t.grad, r.dir);
+    if (!t.valid || !(fabsf(den) > 1e-6f) || !(fabsf(t.value) < 1e-3f)) return false;
+    const float3 w = make_float3(p.x / vsInv, p.y / vsInv, p.z / vsInv);
+    const float3 v = xform(Tinv, w);
+    z = v.z;
+    // tangents of the ray start o and direction (voxel units)
+    const float zNear = 0.1f, zFar = 5.0f;
+    const float ux = K.ifx * ((float)x - K.cx), uy = K.ify * ((float)y - K.cy);
+    const float3 pn = make_float3(ux * zNear, uy * zNear, zNear), pf = make_float3(ux * zFar, uy * zFar, zFar);
+    float3 be = xform(T, pf);
+    be = make_float3(be.x * vsInv, be.y * vsInv, be.z * vsInv);
+    const float3 dvec = make_float3(be.x - o.x, be.y - o.y, be.z - o.z);
+    const float dn = sqrtf(sum3(dvec.x * dvec.x, dvec.y * dvec.y, dvec.z * dvec.z));
+    const float tt = f3dot(make_float3(p.x - o.x, p.y - o.y, p.z - o.z), r.dir);
+    const float3 c2 = make_float3(T.r02, T.r12, T.r22);  // row 2 of R^T
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const float3 os = f3mul(point_tangent(T, pn, j), vsInv);
+        const float3 oe = f3mul(point_tangent(T, pf, j), vsInv);
+        const float3 tv = make_float3(oe.x - os.x, oe.y - os.y, oe.z - os.z);
+        const float3 dd = dn > 0.f ? f3mul(f3add(tv, f3mul(r.dir, -f3dot(r.dir, tv))), 1.0f / dn)
+                                   : make_float3(0.f, 0.f, 0.f);
+        const float3 q = f3add(os, f3mul(dd, tt));
+        const float3 pd = f3add(q, f3mul(r.dir, -f3dot(t.grad, q) / den));
+        float dz = f3dot(c2, pd) / vsInv;
+        if (j == 2) dz -= 1.0f;
+        if (j == 3) dz -= v.y;
+        if (j == 4) dz += v.x;
+        J[j] = dz;
+    }
+    return true;
+}
+
+// one workgroup: kGnThreads sampled pixels of hypothesis blockIdx.y
+template <int FMT, int DF>
+__global__ __launch_bounds__(kGnThreads) void k_pose_gn(MapView m, DepthImage d, Intrinsics K, const Affine* poses,
+                                                        const Affine* posesInv, int stride, int sw, int sh,
+                                                        float vsInv, float rtStep, float maxRes, float* partials) {
+    __shared__ float s_part[kGnThreads / 64][kGnTerms];
+    const int h = blockIdx.y;
+    const int i = blockIdx.x * kGnThreads + threadIdx.x;
+    float acc[kGnTerms];
+#pragma unroll
+    for (int k = 0; k < kGnTerms; ++k) acc[k] = 0.f;
+    if (i < sw * sh) {
+        const int x = (i % sw) * stride, y = (i / sw) * stride;
+        const float dep = load_depth<DF>(d, x, y);
+        if (dep == dep && dep >= m.depthMin && dep <= m.depthMax) {
+            RayReader<FMT> rd(m);
+            float z, J[6];
+            if (surface_depth(rd, x, y, K, poses[h], posesInv[h], vsInv, rtStep, z, J)) {
+                const float res = z - dep;
+                if (fabsf(res) < maxRes) {
+                    int k = 0;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+#pragma unroll
+                        for (int b = a; b < 6; ++b) acc[k++] = J[a] * J[b];
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) acc[21 + a] = J[a] * res;
+                    acc[27] = res * res;
+                    acc[28] = 1.f;
+                }
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kGnTerms; ++k) {
+        float v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) s_part[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kGnTerms) {
+        float v = 0.f;
+        for (int w = 0; w < kGnThreads / 64; ++w) v += s_part[w][threadIdx.x];
+        partials[((int64_t)h * gridDim.x + blockIdx.x) * kGnTerms + threadIdx.x] = v;
+    }
+}
+
+// exp of a twist (rho, phi) as a 4x4 row-major double matrix
+__device__ inline void se3_exp(const double xi[6], double E[16]) {
+    const double wx = xi[3], wy = xi[4], wz = xi[5];
+    const double th2 = wx * wx + wy * wy + wz * wz, th = sqrt(th2);
+    double A, B, C;  // sin(th)/th, (1-cos)/th^2, (th-sin)/th^3
+    if (th < 1e-6) {
+        A = 1.0 - th2 / 6.0;
+        B = 0.5 - th2 / 24.0;
+        C = 1.0 / 6.0 - th2 / 120.0;
+    } else {
+        A = sin(th) / th;
+        B = (1.0 - cos(th)) / th2;
+        C = (th - sin(th)) / (th2 * th);
+    }
+    const double W[9] = {0, -wz, wy, wz, 0, -wx, -wy, wx, 0};
+    double W2[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double s = 0;
+            for (int c = 0; c < 3; ++c) s += W[a * 3 + c] * W[c * 3 + b];
+            W2[a * 3 + b] = s;
+        }
+    for (int a = 0; a < 3; ++a) {
+        double t = 0;
+        for (int b = 0; b < 3; ++b) {
+            const double I = a == b ? 1.0 : 0.0;
+            E[a * 4 + b] = I + A * W[a * 3 + b] + B * W2[a * 3 + b];
+            t += (I + B * W[a * 3 + b] + C * W2[a * 3 + b]) * xi[b];
+        }
+        E[a * 4 + 3] = t;
+    }
+    E[12] = E[13] = E[14] = 0.0;
+    E[15] = 1.0;
+}
+
+__device__ inline Affine affine_from(const double T[16]) {
+    Affine a;
+    a.r00 = (float)T[0]; a.r01 = (float)T[1]; a.r02 = (float)T[2]; a.t0 = (float)T[3];
+    a.r10 = (float)T[4]; a.r11 = (float)T[5]; a.r12 = (float)T[6]; a.t1 = (float)T[7];
+    a.r20 = (float)T[8]; a.r21 = (float)T[9]; a.r22 = (float)T[10]; a.t2 = (float)T[11];
+    return a;
+}
+
+// Rigid inverse in double: [R^T, -R^T t]
+__device__ inline Affine rigid_inverse(const double T[16]) {
+    double I[16];
+    for (int a = 0; a < 3; ++a) {
+        for (int b = 0; b < 3; ++b) I[a * 4 + b] = T[b * 4 + a];
+        I[a * 4 + 3] = -(T[0 * 4 + a] * T[3] + T[1 * 4 + a] * T[7] + T[2 * 4 + a] * T[11]);
+    }
+    return affine_from(I);
+}
+
+// One thread per hypothesis: reduce the partials, record the cost, solve
+// (J^T J + lambda diag(J^T J)) delta = -J^T r and set T <- T exp(delta).
+__global__ void k_pose_solve(const float* partials, int blocksPerHyp, int nHyp, double* posesD, Affine* poses,
+                             Affine* posesInv, float* cost, double* system, int iter, int iters, int update,
+                             double lambda) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nHyp) return;
+    double a[kGnTerms];
+    for (int k = 0; k < kGnTerms; ++k) a[k] = 0.0;
+    for (int b = 0; b < blocksPerHyp; ++b)
+        for (int k = 0; k < kGnTerms; ++k) a[k] += (double)partials[((int64_t)h * blocksPerHyp + b) * kGnTerms + k];
+    const double n = a[28];
+    for (int q = 0; q < 27; ++q) system[((int64_t)h * (iters + 1) + iter) * 27 + q] = a[q];
+    cost[((int64_t)h * (iters + 1) + iter) * 2 + 0] = n > 0 ? (float)(a[27] / n) : -1.f;
+    cost[((int64_t)h * (iters + 1) + iter) * 2 + 1] = (float)n;
+    if (!update || n < 6) return;
+    double H[36];
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+            H[i * 6 + j] = H[j * 6 + i] = a[k++];
+        }
+    for (int i = 0; i < 6; ++i) H[i * 6 + i] *= 1.0 + lambda;
+    // Cholesky H = L L^T
+    double L[36] = {0};
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = H[i * 6 + j];
+            for (int q = 0; q < j; ++q) s -= L[i * 6 + q] * L[j * 6 + q];
+            if (i == j) {
+                if (!(s > 0.0)) return;  // not positive definite: keep the pose
+                L[i * 6 + i] = sqrt(s);
+            } else {
+                L[i * 6 + j] = s / L[j * 6 + j];
+            }
+        }
+    double y[6], delta[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = -a[21 + i];
+        for (int q = 0; q < i; ++q) s -= L[i * 6 + q] * y[q];
+        y[i] = s / L[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int q = i + 1; q < 6; ++q) s -= L[q * 6 + i] * delta[q];
+        delta[i] = s / L[i * 6 + i];
+    }
+    double E[16], Tn[16];
+    se3_exp(delta, E);
+    double* T = posesD + (int64_t)h * 16;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double s = 0;
+            for (int q = 0; q < 4; ++q) s += T[i * 4 + q] * E[q * 4 + j];
+            Tn[i * 4 + j] = s;
+        }
+    for (int i = 0; i < 16; ++i) T[i] = Tn[i];
+    poses[h] = affine_from(Tn);
+    posesInv[h] = rigid_inverse(Tn);
+}
+
+// row-major double poses -> float Affine pairs
+__global__ void k_pose_load(const double* posesD, Affine* poses, Affine* posesInv, int nHyp) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nHyp) return;
+    poses[h] = affine_from(posesD + (int64_t)h * 16);
+    posesInv[h] = rigid_inverse(posesD + (int64_t)h * 16);
+}
+
+template <int FMT>
+hipError_t launch_gn(const MapView& m, const DepthImage& d, const Intrinsics& K, const Affine* poses,
+                     const Affine* posesInv, int nHyp, int stride, int sw, int sh, float maxRes, float* partials,
+                     int blocksPerHyp, hipStream_t s) {
+    const float vsInv = (float)(1.0 / (double)m.voxelSize);
+    const float rtStep = m.trunc / m.voxelSize;
+    const dim3 grid(blocksPerHyp, nHyp);
+    if (d.format == kDepthU16)
+        hipLaunchKernelGGL((k_pose_gn<FMT, kDepthU16>), grid, dim3(kGnThreads), 0, s, m, d, K, poses, posesInv,
+                           stride, sw, sh, vsInv, rtStep, maxRes, partials);
+    else
+        hipLaunchKernelGGL((k_pose_gn<FMT, kDepthF32>), grid, dim3(kGnThreads), 0, s, m, d, K, poses, posesInv,
+                           stride, sw, sh, vsInv, rtStep, maxRes, partials);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t refine_poses_scratch_bytes(int W, int H, int nHyp, int iters, int stride) {
+    const int sw = (W + stride - 1) / stride, sh = (H + stride - 1) / stride;
+    const int bph = (sw * sh + kGnThreads - 1) / kGnThreads;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return up((size_t)nHyp * bph * kGnTerms * sizeof(float)) + up((size_t)nHyp * 16 * sizeof(double)) +
+           2 * up((size_t)nHyp * sizeof(Affine)) + up((size_t)nHyp * (iters + 1) * 2 * sizeof(float)) +
+           up((size_t)nHyp * (iters + 1) * 27 * sizeof(double));
+}
+
+// posesIn: nHyp row-major 4x4 doubles (host). scratch: refine_poses_scratch_bytes.
+// On return (asynchronous on s) the scratch holds the refined row-major
+// doubles at `posesOut`, the costs at `costOut` and the normal equations of
+// every evaluation at `systemOut`.
+hipError_t launch_refine_poses(const MapView& m, const DepthImage& d, const Intrinsics& K, const double* posesIn,
+                               int nHyp, int iters, int stride, float maxRes, double lambda, void* scratch,
+                               double** posesOut, float** costOut, double** systemOut, hipStream_t s) {
+    const int sw = (d.width + stride - 1) / stride, sh = (d.height + stride - 1) / stride;
+    const int bph = (sw * sh + kGnThreads - 1) / kGnThreads;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    uint8_t* p = (uint8_t*)scratch;
+    float* partials = (float*)p;
+    p += up((size_t)nHyp * bph * kGnTerms * sizeof(float));
+    double* posesD = (double*)p;
+    p += up((size_t)nHyp * 16 * sizeof(double));
+    Affine* poses = (Affine*)p;
+    p += up((size_t)nHyp * sizeof(Affine));
+    Affine* posesInv = (Affine*)p;
+    p += up((size_t)nHyp * sizeof(Affine));
+    float* cost = (float*)p;
+    p += up((size_t)nHyp * (iters + 1) * 2 * sizeof(float));
+    double* system = (double*)p;
+    hipError_t e = hipMemcpyAsync(posesD, posesIn, (size_t)nHyp * 16 * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    const int tb = 64, nb = (nHyp + tb - 1) / tb;
+    hipLaunchKernelGGL(k_pose_load, dim3(nb), dim3(tb), 0, s, posesD, poses, posesInv, nHyp);
+    for (int it = 0; it <= iters; ++it) {
+        switch (m.fmt) {
+            case kFmtRef: e = launch_gn<kFmtRef>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, partials, bph, s); break;
+            case kFmtRefRGB: e = launch_gn<kFmtRefRGB>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, partials, bph, s); break;
+            default: e = launch_gn<kFmtF32>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, partials, bph, s);
+        }
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_pose_solve, dim3(nb), dim3(tb), 0, s, partials, bph, nHyp, posesD, poses, posesInv,
+                           cost, system, it, iters, (int)(it < iters), lambda);
+    }
+    *posesOut = posesD;
+    *costOut = cost;
+    *systemOut = system;
+    return hipGetLastError();
+}
+
+}  // namespace vmamd

@@ -234,6 +234,31 @@ class DeviceMap:
         return {0: 20 * d.n_entry, 1: 4 * d.n_block, 2: self.voxel_bytes * 512 * d.n_block, 3: 64,
                 4: 20 * d.n_entry, 5: 16 * d.n_block}[which]
 
+    def refine_poses(self, depth, K, init_poses, iters: int = 8, stride: int = 4, max_residual: float = 0.05,
+                     depth_scale: float = 1e-3, stream=None, return_system: bool = False):
+        """Gauss-Newton refinement of camera-to-world hypotheses against one
+        depth image (vm_refine_poses). init_poses: (n, 4, 4) matrices or
+        (n, 16) column-major. Returns (poses (n, 4, 4) float32, cost
+        (n, iters + 1, 2) = {mean squared residual, count} per evaluation),
+        plus the normal equations (n, iters + 1, 27) if return_system."""
+        p, pitch, W, H, fmt, sc = self._depth(depth, depth_scale)
+        P = np.asarray(init_poses, np.float32)
+        if P.ndim == 3 and P.shape[1:] == (4, 4):
+            P = np.ascontiguousarray(P.transpose(0, 2, 1).reshape(-1, 16))
+        elif P.ndim != 2 or P.shape[1] != 16:
+            raise ValueError("init_poses must be (n, 4, 4) or (n, 16)")
+        n = P.shape[0]
+        P = np.ascontiguousarray(P)
+        out = np.zeros((n, 16), np.float32)
+        cost = np.zeros((n, iters + 1, 2), np.float32)
+        system = np.zeros((n, iters + 1, 27), np.float64)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        check(lib().vm_refine_poses(self._h, p, pitch, W, H, fmt, sc, intrinsics(K), fp(P), n, iters, stride,
+                                    max_residual, fp(out), fp(cost), system.ctypes.data_as(C.POINTER(C.c_double)),
+                                    _stream(stream)))
+        poses = out.reshape(n, 4, 4).transpose(0, 2, 1).copy()
+        return (poses, cost, system) if return_system else (poses, cost)
+
     def download(self, which: int) -> np.ndarray:
         n = self._nbytes(which)
         out = np.empty(n, np.uint8)

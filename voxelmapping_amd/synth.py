@@ -5,6 +5,7 @@ The workload definitions used by the tests and the benchmark live here:
 * :func:`room_small` -- 4x4x3 m room, 5 spheres, 3 boxes, 320x240 float
   metres depth (fx = fy = 262.5), 50 frames on a smooth circular trajectory,
   1 cm voxels, 3 cm truncation, 2^17 buckets, 2^16-block heap.
+* :func:`room_large` -- 10x10x3 m, 40 primitives, 1280x720, 2 mm voxels;
 * :func:`room_vga` -- the same generator at 640x480 (fx = fy = 525), depth
   as uint16 millimetres with scale 1/1000 and a 4 m cut, 500 frames of
   seeded random walk (<= 2 cm, <= 1 degree per frame), 5 mm voxels, 2 cm
@@ -135,6 +136,20 @@ def room_small(seed: int = 1, **kw) -> Workload:
                 n_frames=50, trajectory=0, depth_u16=False, depth_scale=1.0, max_depth=5.0,
                 voxel_size=0.01, trunc=0.03, n_bucket=1 << 17, n_excess=1 << 14, n_block=1 << 16,
                 voxel_format=0)
+    args.update(kw)
+    return Workload(**args)
+
+
+def room_large(seed: int = 5, **kw) -> Workload:
+    """10 x 10 x 3 m room with 40 primitives (25 spheres, 15 boxes) at
+    1280x720 (fx = fy = 920), uint16 mm depth up to 5 m, 2000-frame random
+    walk, fused at 2 mm with 1 cm truncation into 2^23 buckets (+2^20 excess)
+    and 2^22 float-TSDF blocks (24 GiB of voxels)."""
+    args = dict(name="room_large_1280x720_2mm", seed=seed, width=1280, height=720, K=(920.0, 920.0, 639.5, 359.5),
+                n_frames=2000, trajectory=1, depth_u16=True, depth_scale=1.0 / 1000.0, max_depth=5.0,
+                n_spheres=25, n_boxes=15, room=(10.0, 10.0, 3.0),
+                voxel_size=0.002, trunc=0.01, n_bucket=1 << 23, n_excess=1 << 20, n_block=1 << 22,
+                voxel_format=2)
     args.update(kw)
     return Workload(**args)
 
