@@ -194,6 +194,65 @@ def batched_bench(warmup: int, steps: int) -> dict:
     }
 
 
+def pose_bench(m, w, depth, T, n_hyp: int = 32, iters: int = 8, stride: int = 4, reps: int = 3) -> dict:
+    """Batched pose refinement on the fused room_vga map: n_hyp hypotheses
+    around the frame's pose (U[-2, 2] cm, U[-1.5, 1.5] deg), iters
+    Gauss-Newton iterations over every stride-th pixel of the 640x480 depth."""
+    from voxelmapping_amd.autograd import se3_exp
+    rng = np.random.default_rng(7)
+    T0 = np.asarray(T, np.float64)
+    hyps = np.stack([T0 @ se3_exp(np.r_[rng.uniform(-1, 1, 3) * 0.02, np.radians(rng.uniform(-1, 1, 3) * 1.5)])
+                     for _ in range(n_hyp)]).astype(np.float32)
+    m.refine_poses(depth, w.K, hyps, iters=iters, stride=stride, depth_scale=w.depth_scale)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        poses, cost = m.refine_poses(depth, w.K, hyps, iters=iters, stride=stride, depth_scale=w.depth_scale)
+    dt = (time.perf_counter() - t0) / reps
+    rays = n_hyp * (iters + 1) * ((w.width + stride - 1) // stride) * ((w.height + stride - 1) // stride)
+    best = int(np.argmin(cost[:, -1, 0] + (cost[:, -1, 0] < 0) * 1e9))
+    dT = np.linalg.inv(T0) @ poses[best].astype(np.float64)
+    return {"workload": f"{n_hyp} hypotheses x {iters} Gauss-Newton iterations, 640x480 depth every {stride}th pixel, "
+                        "room_vga map", "ms_per_refinement": round(1000 * dt, 3),
+            "rays_per_s": round(rays / dt, 1), "launches_per_iteration": 1,
+            "best_hypothesis_error_mm": round(1000 * float(np.linalg.norm(dT[:3, 3])), 2)}
+
+
+def scale_bench(frames: int = 24, warmup: int = 4) -> dict:
+    """The large-room workload at full size: 1280x720 uint16 depth fused at
+    2 mm into 2^23 buckets / 2^22 float-TSDF blocks (24 GiB of voxels), fuse +
+    raycast + normals per frame."""
+    import torch
+    from voxelmapping_amd import computeNormal, synth
+    from voxelmapping_amd.mapping import DeviceMap
+    w = synth.room_large()
+    m = DeviceMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, w.voxel_format, 0.1, 5.0)
+    n = warmup + frames
+    deps = [torch.from_numpy(w.frame(i, want_rgb=False)[0].view(np.int16)).cuda().view(torch.uint16)
+            for i in range(n)]
+    poses = w.poses(n)
+    vm = torch.zeros((w.height, w.width, 4), dtype=torch.float32, device="cuda")
+    nm = torch.zeros_like(vm)
+
+    def step(i):
+        m.fuse(deps[i], w.K, poses[i], depth_scale=w.depth_scale)
+        m.render_scene(vm, w.K, poses[i])
+        computeNormal(vm, nm)
+
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(warmup, n):
+        step(i)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = m.stats()
+    m.release()
+    return {"workload": w.name, "frames": [warmup, n], "value": round(frames / dt, 2), "unit": "frames/s",
+            "ms_per_step": round(1000 * dt / frames, 4), "allocated_blocks": st["allocated_blocks"],
+            "status": st["status"], "voxel_pool_gib": round(w.n_block * 512 * 12 / 2**30, 1)}
+
+
 def adjoint_bench(reps: int = 20) -> dict:
     """Raycast forward vs its adjoint: 640x480 rays (fx = fy = 525) from a
     pose perturbed by (1 cm, 0.5 deg) into the fused room_small volume (50
@@ -362,6 +421,9 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_bench(args.batched_warmup, args.batched_steps)
         out["adjoint"] = adjoint_bench()
+        last = args.warmup + args.steps - 1
+        out["pose_refine"] = pose_bench(m, w, depth_d[last].view(torch.uint16), w.poses(nframes)[last].reshape(4, 4).T)
+        out["scale"] = scale_bench()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth.room_vga(), args.cpu_seconds)
     print(json.dumps(out), flush=True)
