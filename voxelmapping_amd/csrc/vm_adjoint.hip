@@ -46,21 +46,46 @@ __device__ inline float3 rotT(const Affine& T, float3 v) {
                        T.r02 * v.x + T.r12 * v.y + T.r22 * v.z);
 }
 
+// dTSDF contributions of one wave, summed per voxel in LDS (neighbouring rays
+// touch the same voxels) and added to global memory once per voxel by
+// flush(). An open-addressed table of kAccSlots voxel indices; a contribution
+// that finds no slot within kAccProbes goes to global memory at once.
+constexpr int kAccSlots = 512;
+constexpr int kAccProbes = 8;
+struct GradAcc {
+    int* keys;     // LDS, -1 = free
+    float* vals;   // LDS
+    float* g;      // global dTSDF
+    __device__ inline void add(int64_t idx, float v) {
+        const int key = (int)idx;  // voxel slots of the float format are < 2^31
+        const uint32_t h = ((uint32_t)key * 2654435761u) >> 23;  // 9 bits
+        for (int k = 0; k < kAccProbes; ++k) {
+            const int slot = (int)((h + k) & (kAccSlots - 1));
+            const int prev = atomicCAS(keys + slot, -1, key);
+            if (prev == -1 || prev == key) {
+                atomicAdd(vals + slot, v);
+                return;
+            }
+        }
+        atomicAdd(g + idx, v);
+    }
+};
+
 // one reverse step: position q carried a sample s(q) used as rtStep * s(q) * dir
 // with multiplier lam; returns the added position adjoint, scatters dTSDF
 __device__ inline float3 reverse_sample(RayReader<kFmtF32>& rd, float3 q, bool interp, float3 lam, float3 dir,
-                                        float rtStep, float* gTsdf) {
+                                        float rtStep, GradAcc& acc) {
     const float a = dot3f(lam, dir) * rtStep;
     if (interp) {
         Trilinear t;
         trilinear(rd, q, t);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (t.ok[k] && t.w[k] != 0.f) atomicAdd(gTsdf + t.idx[k], a * t.w[k]);
+            if (t.ok[k] && t.w[k] != 0.f) acc.add(t.idx[k], a * t.w[k]);
         return mul3(t.grad, a);
     }
     const int64_t i = rd.voxel(floor_i(q.x), floor_i(q.y), floor_i(q.z));
-    if (i >= 0) atomicAdd(gTsdf + i, a);
+    if (i >= 0) acc.add(i, a);
     return make_float3(0.f, 0.f, 0.f);
 }
 
@@ -69,19 +94,29 @@ struct StackRec {
     int meta;  // bit 0: interp sample; bits 1..: unit steps after this step
 };
 
-template <int TILE>
-__global__ __launch_bounds__(256) void k_raycast_backward(MapView m, const float* gv, int64_t gpitch, int W, int H,
-                                                          Intrinsics K, Affine pose, Affine Tinv, float vsInv,
-                                                          float rtStep, float* gTsdf, float* poseParts,
-                                                          StackRec* stack, int* stats) {
-    __shared__ float s_red[8][6];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tilesX = (W + 15) / 16;
-    const int x = (blockIdx.x % tilesX) * 16 + (threadIdx.x & 15);
-    const int y = (blockIdx.x / tilesX) * 16 + (threadIdx.x >> 4);
+// One wave per 8x8-pixel tile, tiles in the raycast's XCD-banded order; each
+// wave writes its 6 pose parts.
+__global__ __launch_bounds__(64) void k_raycast_backward(MapView m, const float* gv, int64_t gpitch, int W, int H,
+                                                         Intrinsics K, Affine pose, Affine Tinv, float vsInv,
+                                                         float rtStep, float* gTsdf, float* poseParts,
+                                                         StackRec* stack, int* stats, int tilesX, int tilesPerXcd,
+                                                         int nTiles) {
+    __shared__ int s_keys[kAccSlots];
+    __shared__ float s_vals[kAccSlots];
+    const int lane = threadIdx.x;
+    for (int i = lane; i < kAccSlots; i += 64) {
+        s_keys[i] = -1;
+        s_vals[i] = 0.f;
+    }
+    __syncthreads();
+    GradAcc acc{s_keys, s_vals, gTsdf};
+    const int g = blockIdx.x;
+    const int tile = (g & 7) * tilesPerXcd + (g >> 3);
+    const int x = (tile % tilesX) * 8 + (lane & 7);
+    const int y = (tile / tilesX) * 8 + (lane >> 3);
     float part[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool overflow = false, hit = false;
-    if (x < W && y < H) {
+    if (tile < nTiles && x < W && y < H) {
         const float4 g4 = *((const float4*)((const uint8_t*)gv + (int64_t)y * gpitch) + x);
         const float3 gvec = make_float3(g4.x, g4.y, g4.z);
         if (gvec.x != 0.f || gvec.y != 0.f || gvec.z != 0.f) {
@@ -176,32 +211,37 @@ __global__ __launch_bounds__(256) void k_raycast_backward(MapView m, const float
                         const float a = dot3f(lam, dir) * rtStep;
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
-                            if (ta.ok[k] && ta.w[k] != 0.f) atomicAdd(gTsdf + ta.idx[k], a * ta.w[k]);
+                            if (ta.ok[k] && ta.w[k] != 0.f) acc.add(ta.idx[k], a * ta.w[k]);
                         gDir = add3(gDir, mul3(lam, rtStep * sb));
                         lam = add3(lam, mul3(ta.grad, a));
                     }
                     // r_a = r_k* + rtStep s_k* dir
                     gDir = add3(gDir, mul3(lam, rtStep * sk));
-                    lam = add3(lam, reverse_sample(rd, rk, kInterp, lam, dir, rtStep, gTsdf));
+                    lam = add3(lam, reverse_sample(rd, rk, kInterp, lam, dir, rtStep, acc));
                     // unit steps before the break, then the active steps in reverse
                     gDir = add3(gDir, mul3(lam, (float)units));
                     for (int d = depth - 1; d >= 0; --d) {
                         const StackRec rec = stack[(int64_t)d * P + pix];
                         const float3 q = make_float3(rec.x, rec.y, rec.z);
-                        Trilinear t;
-                        float s;
+                        // the sample is read once: its value for d/d dir, its
+                        // weights and gradient for dTSDF and lambda
+                        const float a = dot3f(lam, dir) * rtStep;
                         if (rec.meta & 1) {
+                            Trilinear t;
                             trilinear(rd, q, t);
-                            s = t.value;
+                            gDir = add3(gDir, mul3(lam, t.value * rtStep + (float)(rec.meta >> 1)));
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                if (t.ok[k] && t.w[k] != 0.f) acc.add(t.idx[k], a * t.w[k]);
+                            lam = add3(lam, mul3(t.grad, a));
                         } else {
                             float v = 1.f;
                             bool ok = false;
                             const int64_t i = rd.voxel(floor_i(q.x), floor_i(q.y), floor_i(q.z));
                             if (i >= 0) Vox<kFmtF32>::read(m.voxels, i, v, ok);
-                            s = v;
+                            gDir = add3(gDir, mul3(lam, v * rtStep + (float)(rec.meta >> 1)));
+                            if (i >= 0) acc.add(i, a);
                         }
-                        gDir = add3(gDir, mul3(lam, s * rtStep + (float)(rec.meta >> 1)));
-                        lam = add3(lam, reverse_sample(rd, q, rec.meta & 1, lam, dir, rtStep, gTsdf));
                     }
                     gDir = add3(gDir, mul3(lam, (float)(depth > 0 ? lead : 0)));
                     // dir = dvec / |dvec|; bs gets lam - d/d dvec, be gets d/d dvec
@@ -218,22 +258,22 @@ __global__ __launch_bounds__(256) void k_raycast_backward(MapView m, const float
             }
         }
     }
-    // workgroup reduction of the pose parts into one slot per workgroup
+    // the wave's dTSDF sums, one global atomic per voxel
+    __syncthreads();
+    for (int i = lane; i < kAccSlots; i += 64)
+        if (s_keys[i] >= 0) atomicAdd(gTsdf + s_keys[i], s_vals[i]);
+    // wave reduction of the pose parts into this wave's slot
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         float v = part[k];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        if (lane == 0) s_red[wave][k] = v;
+        if (lane == 0) poseParts[(int64_t)blockIdx.x * 6 + k] = v;
     }
     const unsigned long long hits = __ballot(hit), ovf = __ballot(overflow);
     if (lane == 0) {
         if (hits) atomicAdd(&stats[0], __popcll(hits));
         if (ovf) atomicAdd(&stats[1], __popcll(ovf));
     }
-    __syncthreads();
-    if (threadIdx.x < 6)
-        poseParts[(int64_t)blockIdx.x * 6 + threadIdx.x] =
-            s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] + s_red[3][threadIdx.x];
 }
 
 __global__ void k_pose_reduce(const float* parts, int n, double* gPose) {
@@ -301,10 +341,12 @@ __global__ void k_normals_backward(const float* vmap, int64_t vpitch, const floa
 }  // namespace
 
 // scratch layout: [stack | pose parts (256-byte aligned) | vertex gradients]
-static size_t parts_bytes(int W, int H) {
-    const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
-    return ((sizeof(float) * 6 * tiles + 255) / 256) * 256;
+// one launch slot per 8x8 tile, rounded up to whole XCD bands
+static size_t backward_waves(int W, int H) {
+    const size_t tiles = (size_t)((W + 7) / 8) * ((H + 7) / 8);
+    return (tiles + 7) / 8 * 8;
 }
+static size_t parts_bytes(int W, int H) { return ((sizeof(float) * 6 * backward_waves(W, H) + 255) / 256) * 256; }
 
 size_t raycast_backward_scratch_bytes(int W, int H) {
     return sizeof(StackRec) * kStackDepth * (size_t)W * H + parts_bytes(W, H) + sizeof(float) * 4 * (size_t)W * H;
@@ -314,7 +356,8 @@ hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t 
                                    int64_t gvpitch, const float* gNormal, int64_t gnpitch, int W, int H,
                                    const Intrinsics& K, const Affine& pose, const Affine& Tinv, float* gTsdf,
                                    double* gPose, void* scratch, int* stats, hipStream_t s) {
-    const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
+    const int tilesX = (W + 7) / 8, nTiles = tilesX * ((H + 7) / 8);
+    const int waves = (int)backward_waves(W, H);
     uint8_t* p = (uint8_t*)scratch;
     StackRec* stack = (StackRec*)p;
     p += sizeof(StackRec) * kStackDepth * (size_t)W * H;
@@ -326,9 +369,9 @@ hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t 
                        gvpitch, gNormal, gnpitch, gv, gpitch, W, H);
     const float vsInv = (float)(1.0 / (double)m.voxelSize);
     const float rtStep = m.trunc / m.voxelSize;
-    hipLaunchKernelGGL(k_raycast_backward<16>, dim3((unsigned)tiles), dim3(256), 0, s, m, gv, gpitch, W, H, K, pose,
-                       Tinv, vsInv, rtStep, gTsdf, parts, stack, stats);
-    hipLaunchKernelGGL(k_pose_reduce, dim3(6), dim3(256), 0, s, parts, (int)tiles, gPose);
+    hipLaunchKernelGGL(k_raycast_backward, dim3((unsigned)waves), dim3(64), 0, s, m, gv, gpitch, W, H, K, pose, Tinv,
+                       vsInv, rtStep, gTsdf, parts, stack, stats, tilesX, waves / 8, nTiles);
+    hipLaunchKernelGGL(k_pose_reduce, dim3(6), dim3(256), 0, s, parts, waves, gPose);
     return hipGetLastError();
 }
 
