@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void kb_integrate(BatchView B) {
 // ray tiles of every scene, concatenated; tile shape tw x (64 / tw); with
 // xcdBands the flat tile index is remapped so each XCD gets a contiguous range
 template <int FMT>
-__global__ __launch_bounds__(64) void kb_raycast(BatchView B, int tw, int xcdBands) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void kb_raycast(BatchView B, int tw, int xcdBands) {
     __shared__ int s_first[kMaxScenes];
     for (int k = threadIdx.x; k < B.nScenes; k += blockDim.x) s_first[k] = B.scenes[k].rayTileFirst;
     __syncthreads();

@@ -195,101 +195,140 @@ __device__ inline bool block_visible(const MapView& m, int bx, int by, int bz, c
 }
 
 // Integration of one visible block by one wave: lane = (x, y), 8 z-slices
-// (DepthFusionFunctor :148-196 / ImageFusionFunctor :328-380). All eight
-// projections and depth tests are done before the voxel loads, so a lane
-// has its eight read-modify-writes in flight together.
+// (DepthFusionFunctor :148-196 / ImageFusionFunctor :328-380).
+//
+// Every read is issued before any of its results is needed, so a lane has
+// two memory round trips instead of one or two per slice: the eight depth
+// reads (a projection outside the image reads pixel (0, 0), unused), then
+// the eight voxels of the lane's column and the eight colour pixels
+// together. Voxels whose depth test fails are read but never written.
 template <int FMT, int DF, bool COLOR>
 __device__ inline void integrate_block(const MapView& m, const DepthImage& d, const ColorImage& c,
                                        const Intrinsics& K, const Affine& Tinv, int blk, int4 key, int lane) {
     const int lx = lane & 7, ly = lane >> 3;
     const float vs = m.voxelSize, tr = m.trunc;
     const int vx = key.x * 8 + lx, vy = key.y * 8 + ly, vz0 = key.z * 8;
-    float sdfNew[8];
+    float dist[8], pz[8];
     int pix[8];
-    unsigned ok = 0;
+    unsigned inside = 0;
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        float3 p = xform(Tinv, make_float3((float)vx * vs, (float)vy * vs, (float)(vz0 + z) * vs));
+        const float3 p = xform(Tinv, make_float3((float)vx * vs, (float)vy * vs, (float)(vz0 + z) * vs));
         // __float2int_rd(fx * x / z + cx + 0.5): the double add rounds the exact
         // sum once to float, exactly like a float add
         const int u = floor_i(K.fx * p.x / p.z + K.cx + 0.5f);
         const int v = floor_i(K.fy * p.y / p.z + K.cy + 0.5f);
-        pix[z] = 0;
-        sdfNew[z] = 0.f;
-        if (u < 0 || v < 0 || u > d.width - 1 || v > d.height - 1) continue;
-        const float dist = load_depth<DF>(d, u, v);
-        if (isnan(dist) || dist > m.depthMax || dist < m.depthMin) continue;
-        float sdf = dist - p.z;
-        if (sdf < -tr) continue;
-        sdfNew[z] = fminf(1.0f, sdf / tr);
-        pix[z] = v * 65536 + u;
-        ok |= 1u << z;
+        const bool in = !(u < 0 || v < 0 || u > d.width - 1 || v > d.height - 1);
+        const int uc = in ? u : 0, vc = in ? v : 0;
+        inside |= (unsigned)in << z;
+        pix[z] = vc * 65536 + uc;
+        pz[z] = p.z;
+        dist[z] = load_depth<DF>(d, uc, vc);
     }
-    uint8_t* base = m.voxels + ((int64_t)blk * kBlockVox + lane) * Vox<FMT>::kBytes;
-    constexpr int64_t kSlice = 64 * Vox<FMT>::kBytes;
+    float sdfNew[8];
+    unsigned ok = 0;
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        if (!(ok & (1u << z))) continue;
-        uint8_t* q = base + z * kSlice;
-        const float sdf = sdfNew[z];
-        uint8_t cr = 0, cg = 0, cb = 0;
+        const float dd = dist[z];
+        const float sdf = dd - pz[z];
+        const bool good = ((inside >> z) & 1u) && !(isnan(dd) || dd > m.depthMax || dd < m.depthMin) && !(sdf < -tr);
+        sdfNew[z] = fminf(1.0f, sdf / tr);
+        ok |= (unsigned)good << z;
+    }
+    if (ok == 0) return;
+    uint32_t rgb[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        rgb[z] = 0;
         if constexpr (COLOR) {
             const int u = pix[z] & 0xffff, v = pix[z] >> 16;
             const uint8_t* cp = c.data + (int64_t)v * c.pitch + u * 3;
-            cr = cp[0]; cg = cp[1]; cb = cp[2];
+            rgb[z] = (uint32_t)cp[0] | ((uint32_t)cp[1] << 8) | ((uint32_t)cp[2] << 16);
         }
-        if constexpr (FMT == kFmtF32) {
-            float* f = (float*)q;
-            const float os = f[0], ow = f[1];
+    }
+    uint8_t* base = m.voxels + ((int64_t)blk * kBlockVox + lane) * Vox<FMT>::kBytes;
+    constexpr int64_t kSlice = 64 * Vox<FMT>::kBytes;
+    if constexpr (FMT == kFmtF32) {
+        float os[8], ow[8];
+        uint32_t oc[8];
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            const float* f = (const float*)(base + z * kSlice);
+            os[z] = f[0];
+            ow[z] = f[1];
+            oc[z] = COLOR ? ((const uint32_t*)f)[2] : 0u;
+        }
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            if (!((ok >> z) & 1u)) continue;
+            float* f = (float*)(base + z * kSlice);
+            const float sdf = sdfNew[z];
             float ns, nw;
-            if (ow == 0.f) { ns = sdf; nw = 1.f; }
-            else { ns = (os * ow + sdf * 1.0f) / (ow + 1.0f); nw = fminf(m.maxWeight, ow + 1.0f); }
+            if (ow[z] == 0.f) { ns = sdf; nw = 1.f; }
+            else { ns = (os[z] * ow[z] + sdf * 1.0f) / (ow[z] + 1.0f); nw = fminf(m.maxWeight, ow[z] + 1.0f); }
             f[0] = ns;
             f[1] = nw;
             if constexpr (COLOR) {
-                uint32_t* cw = (uint32_t*)(q + 8);
-                uint32_t old = *cw;
-                uint32_t r = cr, g = cg, b = cb;
-                if (ow != 0.f) {
-                    r = (uint32_t)(0.7f * (float)(old & 0xffu) + 0.3f * (float)cr);
-                    g = (uint32_t)(0.7f * (float)((old >> 8) & 0xffu) + 0.3f * (float)cg);
-                    b = (uint32_t)(0.7f * (float)((old >> 16) & 0xffu) + 0.3f * (float)cb);
+                const uint32_t old = oc[z];
+                uint32_t r = rgb[z] & 0xffu, g = (rgb[z] >> 8) & 0xffu, b = rgb[z] >> 16;
+                if (ow[z] != 0.f) {
+                    r = (uint32_t)(0.7f * (float)(old & 0xffu) + 0.3f * (float)r);
+                    g = (uint32_t)(0.7f * (float)((old >> 8) & 0xffu) + 0.3f * (float)g);
+                    b = (uint32_t)(0.7f * (float)((old >> 16) & 0xffu) + 0.3f * (float)b);
                 }
-                *cw = (old & 0xff000000u) | r | (g << 8) | (b << 16);
+                ((uint32_t*)f)[2] = (old & 0xff000000u) | r | (g << 8) | (b << 16);
             }
-        } else if constexpr (FMT == kFmtRef) {
-            uint32_t* w32 = (uint32_t*)q;
-            const uint32_t old = *w32;
+        }
+    } else if constexpr (FMT == kFmtRef) {
+        uint32_t ov[8];
+#pragma unroll
+        for (int z = 0; z < 8; ++z) ov[z] = *(const uint32_t*)(base + z * kSlice);
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            if (!((ok >> z) & 1u)) continue;
+            const uint32_t old = ov[z];
             const float os = (float)(int16_t)(old & 0xffffu) / 32767.0f;
             const int ow = (old >> 16) & 0xff;
+            const float sdf = sdfNew[z];
             float ns;
             int nw;
             if (ow == 0) { ns = sdf; nw = 1; }
             else { ns = (os * (float)ow + sdf * 1.0f) / (float)(ow + 1); nw = min(255, ow + 1); }
             const uint32_t ps = (uint16_t)(int16_t)f2i_sat(ns * 32767.0f);
-            *w32 = (old & 0xff000000u) | ((uint32_t)nw << 16) | ps;
-        } else {  // kFmtRefRGB
-            uint16_t* h = (uint16_t*)q;
-            const uint16_t h1 = h[1];
-            const float os = (float)(int16_t)h[0] / 32767.0f;
-            const int ow = h1 & 0xff;
+            *(uint32_t*)(base + z * kSlice) = (old & 0xff000000u) | ((uint32_t)nw << 16) | ps;
+        }
+    } else {  // kFmtRefRGB
+        uint16_t h0[8], h1[8], h2[8];
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            const uint16_t* h = (const uint16_t*)(base + z * kSlice);
+            h0[z] = h[0];
+            h1[z] = h[1];
+            h2[z] = COLOR ? h[2] : (uint16_t)0;
+        }
+#pragma unroll
+        for (int z = 0; z < 8; ++z) {
+            if (!((ok >> z) & 1u)) continue;
+            uint16_t* h = (uint16_t*)(base + z * kSlice);
+            const float os = (float)(int16_t)h0[z] / 32767.0f;
+            const int ow = h1[z] & 0xff;
+            const float sdf = sdfNew[z];
             float ns;
             int nw;
             if (ow == 0) { ns = sdf; nw = 1; }
             else { ns = (os * (float)ow + sdf * 1.0f) / (float)(ow + 1); nw = min(255, ow + 1); }
             h[0] = (uint16_t)(int16_t)f2i_sat(ns * 32767.0f);
             if constexpr (COLOR) {
-                const uint16_t h2 = h[2];
-                uint32_t r = cr, g = cg, b = cb;
+                uint32_t r = rgb[z] & 0xffu, g = (rgb[z] >> 8) & 0xffu, b = rgb[z] >> 16;
                 if (ow != 0) {
-                    r = (uint32_t)(0.7f * (float)(h1 >> 8) + 0.3f * (float)cr);
-                    g = (uint32_t)(0.7f * (float)(h2 & 0xffu) + 0.3f * (float)cg);
-                    b = (uint32_t)(0.7f * (float)(h2 >> 8) + 0.3f * (float)cb);
+                    r = (uint32_t)(0.7f * (float)(h1[z] >> 8) + 0.3f * (float)r);
+                    g = (uint32_t)(0.7f * (float)(h2[z] & 0xffu) + 0.3f * (float)g);
+                    b = (uint32_t)(0.7f * (float)(h2[z] >> 8) + 0.3f * (float)b);
                 }
                 h[1] = (uint16_t)(nw | (r << 8));
                 h[2] = (uint16_t)(g | (b << 8));
             } else {
-                h[1] = (uint16_t)((h1 & 0xff00u) | (uint32_t)nw);
+                h[1] = (uint16_t)((h1[z] & 0xff00u) | (uint32_t)nw);
             }
         }
     }

@@ -16,7 +16,7 @@ namespace vmamd {
 namespace {
 
 template <int FMT>
-__global__ __launch_bounds__(64) void k_raycast(MapView m, float* vmap, int64_t pitch, int W, int H,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_raycast(MapView m, float* vmap, int64_t pitch, int W, int H,
                                                 Intrinsics K, Affine pose, Affine Tinv, float vsInv, float rtStep,
                                                 int gate, int tilesX, int tilesPerXcd, int nTiles) {
     const int g = blockIdx.x;
