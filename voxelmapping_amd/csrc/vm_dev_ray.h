@@ -11,7 +11,8 @@
 //   * inside a block that is not allocated every sample reads "invalid, 1.0"
 //     and steps exactly one voxel (res += 1.0f * dir == res + dir), so those
 //     iterations run as a tight loop of additions with a float-bounds test for
-//     leaving the block, and no reads at all.
+//     leaving the block (or the whole 4x4x4-block brick when none of its
+//     blocks is present), and no reads at all.
 #pragma once
 
 #include "vm_common.h"
@@ -71,6 +72,12 @@ struct RayReader {
     // true: block (bx, by, bz) is in the brick held and its bit is clear
     __device__ inline bool known_absent(int bx, int by, int bz) const {
         return (bx >> 2) == qx && (by >> 2) == qy && (bz >> 2) == qz && (word & brick_bit(bx, by, bz)) == 0;
+    }
+
+    // true: the brick held is that of block (bx, by, bz) and none of its 64
+    // blocks is present
+    __device__ inline bool brick_empty(int bx, int by, int bz) const {
+        return (bx >> 2) == qx && (by >> 2) == qy && (bz >> 2) == qz && word == 0ull;
     }
 
     __device__ inline int block(int bx, int by, int bz) {
@@ -224,20 +231,25 @@ template <int FMT>
 __device__ inline int empty_walk(RayReader<FMT>& rd, float3& res, const float3 dir, const float3 inv, float& dist_s,
                                  const float distEnd, const int bx, const int by, const int bz) {
     // The walk goes on across a face into the next block when that block's
-    // bit in the brick word the lane holds is clear.
+    // bit in the brick word the lane holds is clear. The box known to be
+    // empty is the current block, or its whole brick (32^3 voxels) when the
+    // brick's word is zero.
     int cbx = bx, cby = by, cbz = bz, steps = 0;
     while (true) {
-        const float lx = (float)(cbx * 8), ly = (float)(cby * 8), lz = (float)(cbz * 8);
-        const float hx = lx + 8.f, hy = ly + 8.f, hz = lz + 8.f;
-        // Unit steps that stay strictly inside the block and below distEnd
+        const bool brick = rd.brick_empty(cbx, cby, cbz);
+        const float edge = brick ? 32.f : 8.f;
+        const float lx = (float)((brick ? cbx & ~3 : cbx) * 8), ly = (float)((brick ? cby & ~3 : cby) * 8),
+                    lz = (float)((brick ? cbz & ~3 : cbz) * 8);
+        const float hx = lx + edge, hy = ly + edge, hz = lz + edge;
+        // Unit steps that stay strictly inside the box and below distEnd
         // whatever the rounding of the additions: k <= 64 additions to
-        // coordinates below B move them at most 64 * 2^-24 * B from the exact
-        // k * dir, which the margins cover. They are the same additions as
-        // the checked steps, without the checks.
+        // coordinates below B = max(|l|, |h|) + 1 move them at most
+        // 64 * 2^-24 * B from the exact k * dir, which the margins cover. They
+        // are the same additions as the checked steps, without the checks.
         {
-            const float ex = (fabsf(lx) + 9.f) * 0x1p-18f + 0x1p-16f;
-            const float ey = (fabsf(ly) + 9.f) * 0x1p-18f + 0x1p-16f;
-            const float ez = (fabsf(lz) + 9.f) * 0x1p-18f + 0x1p-16f;
+            const float ex = (fmaxf(fabsf(lx), fabsf(hx)) + 1.f) * 0x1p-18f + 0x1p-16f;
+            const float ey = (fmaxf(fabsf(ly), fabsf(hy)) + 1.f) * 0x1p-18f + 0x1p-16f;
+            const float ez = (fmaxf(fabsf(lz), fabsf(hz)) + 1.f) * 0x1p-18f + 0x1p-16f;
             const float kx = ((dir.x > 0.f ? hx - res.x : res.x - lx) - ex) * inv.x;
             const float ky = ((dir.y > 0.f ? hy - res.y : res.y - ly) - ey) * inv.y;
             const float kz = ((dir.z > 0.f ? hz - res.z : res.z - lz) - ez) * inv.z;
