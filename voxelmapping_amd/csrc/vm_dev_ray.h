@@ -11,8 +11,8 @@
 //   * inside a block that is not allocated every sample reads "invalid, 1.0"
 //     and steps exactly one voxel (res += 1.0f * dir == res + dir), so those
 //     iterations run as a tight loop of additions with a float-bounds test for
-//     leaving the block (or the whole 4x4x4-block brick when none of its
-//     blocks is present), and no reads at all.
+//     leaving the block (or the brick octant or whole 4x4x4-block brick
+//     when none of their blocks is present), and no reads at all.
 #pragma once
 
 #include "vm_common.h"
@@ -74,10 +74,9 @@ struct RayReader {
         return (bx >> 2) == qx && (by >> 2) == qy && (bz >> 2) == qz && (word & brick_bit(bx, by, bz)) == 0;
     }
 
-    // true: the brick held is that of block (bx, by, bz) and none of its 64
-    // blocks is present
-    __device__ inline bool brick_empty(int bx, int by, int bz) const {
-        return (bx >> 2) == qx && (by >> 2) == qy && (bz >> 2) == qz && word == 0ull;
+    // true: the brick word held is that of block (bx, by, bz)
+    __device__ inline bool holds(int bx, int by, int bz) const {
+        return (bx >> 2) == qx && (by >> 2) == qy && (bz >> 2) == qz;
     }
 
     __device__ inline int block(int bx, int by, int bz) {
@@ -232,14 +231,19 @@ __device__ inline int empty_walk(RayReader<FMT>& rd, float3& res, const float3 d
                                  const float distEnd, const int bx, const int by, const int bz) {
     // The walk goes on across a face into the next block when that block's
     // bit in the brick word the lane holds is clear. The box known to be
-    // empty is the current block, or its whole brick (32^3 voxels) when the
-    // brick's word is zero.
+    // empty is the largest of: the whole brick (32^3 voxels) when its word is
+    // zero, the brick octant (2x2x2 blocks, bit mask 0x330033 shifted to the
+    // octant) holding the current block when its eight bits are clear, the
+    // current block.
     int cbx = bx, cby = by, cbz = bz, steps = 0;
     while (true) {
-        const bool brick = rd.brick_empty(cbx, cby, cbz);
-        const float edge = brick ? 32.f : 8.f;
-        const float lx = (float)((brick ? cbx & ~3 : cbx) * 8), ly = (float)((brick ? cby & ~3 : cby) * 8),
-                    lz = (float)((brick ? cbz & ~3 : cbz) * 8);
+        const bool held = rd.holds(cbx, cby, cbz);
+        const bool brick = held && rd.word == 0ull;
+        const bool oct = held && (rd.word & (0x330033ull << (((cbx >> 1) & 1) * 2 + ((cby >> 1) & 1) * 8 +
+                                                            ((cbz >> 1) & 1) * 32))) == 0ull;
+        const float edge = brick ? 32.f : oct ? 16.f : 8.f;
+        const int am = brick ? ~3 : oct ? ~1 : ~0;  // block coordinates of the box's low corner
+        const float lx = (float)((cbx & am) * 8), ly = (float)((cby & am) * 8), lz = (float)((cbz & am) * 8);
         const float hx = lx + edge, hy = ly + edge, hz = lz + edge;
         // Unit steps that stay strictly inside the box and below distEnd
         // whatever the rounding of the additions: k <= 64 additions to
