@@ -195,9 +195,11 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
  * against one depth image (device, formats as vm_fuse_depth) by `iters`
  * Gauss-Newton iterations on the depth residual r = z_render(T) - depth over
  * pixels (x, y) with x and y multiples of `stride`. z_render is the camera z
- * of vm_raycast's vertex; its Jacobian by the twist xi of T exp(xi) is the
- * exact derivative of that march (forward mode, the raycast adjoint's
- * derivative). Residuals with |r| >= max_residual metres, misses and depth
+ * of the map's zero level along the pixel's ray: vm_raycast's hit refined by
+ * three Newton steps on the trilinear TSDF. Its Jacobian by the twist xi of
+ * T exp(xi) is the implicit-surface derivative dt = -grad s . (do + t dd) /
+ * (grad s . d) through the ray's origin and direction. Residuals with
+ * |r| >= max_residual metres, misses and depth
  * outside [depth_min, depth_max] are left out. Each iteration is one launch
  * over all hypotheses; the 6x6 normal equations (Marquardt damping 1e-3) are
  * solved in double and T <- T exp(delta). out_poses (HOST, n_hyp * 16,
