@@ -24,14 +24,16 @@ __device__ inline void st(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_REL
 
 // Lock-free lookup. Slots are keyed once and never re-keyed, so a matching
 // position identifies the slot for good; ptr >= 0 says it is published.
-// Returns the slot, -1 when absent, -2 when the chain bound was hit.
+// Returns the slot, -1 when absent, -2 when the chain bound was hit. The five
+// fields of an entry are read together (one memory round trip per entry;
+// relaxed loads of different words are unordered either way).
 __device__ inline int probe(const MapView& m, int x, int y, int z, int h) {
     int slot = h;
     int guard = m.nEntry - m.nBucket + 1;
     while (true) {
         HashEntry* e = m.table + slot;
-        if (ld(&e->x) == x && ld(&e->y) == y && ld(&e->z) == z && ld(&e->ptr) >= 0) return slot;
-        int off = ld(&e->offset);
+        const int ex = ld(&e->x), ey = ld(&e->y), ez = ld(&e->z), p = ld(&e->ptr), off = ld(&e->offset);
+        if ((ex == x) & (ey == y) & (ez == z) & (p >= 0)) return slot;
         if (off < 0) return -1;
         if (--guard < 0) {
             atomicOr(&m.ctr[kStatus], kStatusChainBound);
@@ -71,10 +73,9 @@ __device__ inline bool locked_insert(const MapView& m, int x, int y, int z, int 
     int guard = m.nEntry - m.nBucket + 1;
     while (true) {
         HashEntry* e = m.table + slot;
-        int p = ld(&e->ptr);
-        if (p >= 0 && ld(&e->x) == x && ld(&e->y) == y && ld(&e->z) == z) return true;
+        const int p = ld(&e->ptr), ex = ld(&e->x), ey = ld(&e->y), ez = ld(&e->z), off = ld(&e->offset);
+        if ((p >= 0) & (ex == x) & (ey == y) & (ez == z)) return true;
         if (p == -1 && empty < 0) empty = slot;
-        int off = ld(&e->offset);
         if (off < 0) { tail = slot; break; }
         if (--guard < 0) {
             atomicOr(&m.ctr[kStatus], kStatusChainBound);
