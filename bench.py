@@ -288,7 +288,7 @@ def adjoint_bench(reps: int = 20) -> dict:
         m.raycast(vm, K, T)
         computeNormal(vm, nm)
         ev[1].record()
-        m.raycast_backward(vm, K, T, gt, gp, grad_vertex=gv, grad_normal=gn)
+        m.raycast_backward(vm, K, T, gt, gp, grad_vertex=gv, grad_normal=gn, stats=False)
         ev[2].record()
         torch.cuda.synchronize()
         fwd += ev[0].elapsed_time(ev[1])

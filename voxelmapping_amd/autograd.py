@@ -57,7 +57,7 @@ class RaycastFunction(torch.autograd.Function):
         gp = torch.zeros(6, dtype=torch.float64, device=vmap.device)
         gv = None if gvmap is None else gvmap.contiguous()
         gn = None if gnmap is None else gnmap.contiguous()
-        dmap.raycast_backward(vmap, ctx.K, ctx.T, gt, gp, grad_vertex=gv, grad_normal=gn)
+        dmap.raycast_backward(vmap, ctx.K, ctx.T, gt, gp, grad_vertex=gv, grad_normal=gn, stats=False)
         return gt, gp.to(device=ctx.xi_device, dtype=torch.float32), None, None, None, None, None
 
 

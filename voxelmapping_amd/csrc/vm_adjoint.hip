@@ -96,7 +96,7 @@ struct StackRec {
 
 // One wave per 8x8-pixel tile, tiles in the raycast's XCD-banded order; each
 // wave writes its 6 pose parts.
-__global__ __launch_bounds__(64) void k_raycast_backward(MapView m, const float* gv, int64_t gpitch, int W, int H,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_raycast_backward(MapView m, const float* gv, int64_t gpitch, int W, int H,
                                                          Intrinsics K, Affine pose, Affine Tinv, float vsInv,
                                                          float rtStep, float* gTsdf, float* poseParts,
                                                          StackRec* stack, int* stats, int tilesX, int tilesPerXcd,
@@ -149,6 +149,9 @@ __global__ __launch_bounds__(64) void k_raycast_backward(MapView m, const float*
             bool valid = false, interpUsed = false;
             float step;
             int depth = 0, lead = 0, units = 0;  // units: unit steps since the last pushed record
+            // the last pushed record stays in registers until the unit steps
+            // after it are known (the next push, or the end of the march)
+            StackRec pend{0.f, 0.f, 0.f, 0};
             while (dist_s < distEnd) {
                 last = sdf;
                 const int vx = floor_i(res.x), vy = floor_i(res.y), vz = floor_i(res.z);
@@ -172,10 +175,14 @@ __global__ __launch_bounds__(64) void k_raycast_backward(MapView m, const float*
                 step = valid ? fmaxf(sdf * rtStep, 1.0f) : 1.0f;
                 const bool active = valid && sdf * rtStep > 1.0f;
                 if (active) {
-                    if (depth > 0) stack[(int64_t)(depth - 1) * P + pix].meta |= units << 1;
-                    else lead = units;
+                    if (depth > 0) {
+                        pend.meta |= units << 1;
+                        stack[(int64_t)(depth - 1) * P + pix] = pend;
+                    } else {
+                        lead = units;
+                    }
                     if (depth < kStackDepth) {
-                        stack[(int64_t)depth * P + pix] = StackRec{res.x, res.y, res.z, interpUsed ? 1 : 0};
+                        pend = StackRec{res.x, res.y, res.z, interpUsed ? 1 : 0};
                         ++depth;
                     } else {
                         overflow = true;
@@ -187,6 +194,7 @@ __global__ __launch_bounds__(64) void k_raycast_backward(MapView m, const float*
                 res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
                 dist_s += step;
             }
+            if (depth > 0) stack[(int64_t)(depth - 1) * P + pix] = pend;
             if (sdf <= 0.0f && !overflow) {
                 const float3 rk = res;
                 const bool kInterp = interpUsed;

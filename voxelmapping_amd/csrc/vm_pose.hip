@@ -217,27 +217,43 @@ __device__ inline Affine rigid_inverse(const double T[16]) {
     return affine_from(I);
 }
 
-// One thread per hypothesis: reduce the partials, record the cost, solve
-// (J^T J + lambda diag(J^T J)) delta = -J^T r and set T <- T exp(delta).
-__global__ void k_pose_solve(const float* partials, int blocksPerHyp, int nHyp, double* posesD, Affine* poses,
-                             Affine* posesInv, float* cost, double* system, int iter, int iters, int update,
-                             double lambda) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nHyp) return;
+// One workgroup of kSolveThreads per hypothesis: the partials are summed in
+// double by kSolveThreads / 32 groups of 32 threads (one term per thread,
+// every (kSolveThreads / 32)-th block per group) and the groups' sums added in
+// LDS; thread 0 then records the cost, solves
+// (J^T J + lambda diag(J^T J)) delta = -J^T r and sets T <- T exp(delta).
+constexpr int kSolveThreads = 256;
+__global__ __launch_bounds__(kSolveThreads) void k_pose_solve(const float* partials, int blocksPerHyp, int nHyp,
+                                                              double* posesD, Affine* poses, Affine* posesInv,
+                                                              float* cost, double* system, int iter, int iters,
+                                                              int update, double lambda) {
+    constexpr int kGroups = kSolveThreads / 32;
+    __shared__ double s_sum[kGroups][32];
+    const int h = blockIdx.x;
+    const int k = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    double v = 0.0;
+    if (k < kGnTerms)
+        for (int b = grp; b < blocksPerHyp; b += kGroups)
+            v += (double)partials[((int64_t)h * blocksPerHyp + b) * kGnTerms + k];
+    s_sum[grp][k] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double a[kGnTerms];
-    for (int k = 0; k < kGnTerms; ++k) a[k] = 0.0;
-    for (int b = 0; b < blocksPerHyp; ++b)
-        for (int k = 0; k < kGnTerms; ++k) a[k] += (double)partials[((int64_t)h * blocksPerHyp + b) * kGnTerms + k];
+    for (int q = 0; q < kGnTerms; ++q) {
+        double t = 0.0;
+        for (int g = 0; g < kGroups; ++g) t += s_sum[g][q];
+        a[q] = t;
+    }
     const double n = a[28];
     for (int q = 0; q < 27; ++q) system[((int64_t)h * (iters + 1) + iter) * 27 + q] = a[q];
     cost[((int64_t)h * (iters + 1) + iter) * 2 + 0] = n > 0 ? (float)(a[27] / n) : -1.f;
     cost[((int64_t)h * (iters + 1) + iter) * 2 + 1] = (float)n;
     if (!update || n < 6) return;
     double H[36];
-    int k = 0;
+    int e = 0;
     for (int i = 0; i < 6; ++i)
         for (int j = i; j < 6; ++j) {
-            H[i * 6 + j] = H[j * 6 + i] = a[k++];
+            H[i * 6 + j] = H[j * 6 + i] = a[e++];
         }
     for (int i = 0; i < 6; ++i) H[i * 6 + i] *= 1.0 + lambda;
     // Cholesky H = L L^T
@@ -346,8 +362,8 @@ hipError_t launch_refine_poses(const MapView& m, const DepthImage& d, const Intr
             default: e = launch_gn<kFmtF32>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, partials, bph, s);
         }
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_pose_solve, dim3(nb), dim3(tb), 0, s, partials, bph, nHyp, posesD, poses, posesInv,
-                           cost, system, it, iters, (int)(it < iters), lambda);
+        hipLaunchKernelGGL(k_pose_solve, dim3(nHyp), dim3(kSolveThreads), 0, s, partials, bph, nHyp, posesD, poses,
+                           posesInv, cost, system, it, iters, (int)(it < iters), lambda);
     }
     *posesOut = posesD;
     *costOut = cost;

@@ -191,10 +191,12 @@ class DeviceMap:
         return vmap
 
     def raycast_backward(self, vmap, K, cam_to_world, grad_tsdf, grad_pose, grad_vertex=None, grad_normal=None,
-                         stream=None) -> dict:
+                         stream=None, stats=True):
         """Adjoint of raycast + normals (float voxel format): accumulates
         d loss / d TSDF into grad_tsdf (float32, n_block * 512) and d loss / d xi
-        into grad_pose (float64, 6) for camToWorld = T exp(xi)."""
+        into grad_pose (float64, 6) for camToWorld = T exp(xi). With stats,
+        waits for the result and returns {hit_pixels, overflow_pixels};
+        without, returns None and leaves the work queued on the stream."""
         p, pitch, W, H = _image(vmap, 16, "vmap")
         gv, gvp = (None, 0) if grad_vertex is None else _image(grad_vertex, 16, "grad_vertex")[:2]
         gn, gnp = (None, 0) if grad_normal is None else _image(grad_normal, 16, "grad_normal")[:2]
@@ -205,8 +207,9 @@ class DeviceMap:
         st = _capi.BackwardStats()
         check(lib().vm_raycast_backward(self._h, p, pitch, gv, gvp, gn, gnp, W, H, intrinsics(K),
                                         pose_colmajor(cam_to_world), C.c_void_p(grad_tsdf.data_ptr()),
-                                        C.c_void_p(grad_pose.data_ptr()), _stream(stream), C.byref(st)))
-        return {"hit_pixels": st.hit_pixels, "overflow_pixels": st.overflow_pixels}
+                                        C.c_void_p(grad_pose.data_ptr()), _stream(stream),
+                                        C.byref(st) if stats else None))
+        return {"hit_pixels": st.hit_pixels, "overflow_pixels": st.overflow_pixels} if stats else None
 
     def project_rendering_blocks(self, zrange_x, zrange_y, K, world_to_cam, stream=None) -> int:
         px, pitch, W8, H8 = _image(zrange_x, 4, "zrange_x")
