@@ -20,8 +20,12 @@ Pinned here, per oracle function:
   polygonize            all 256 single-cube cases vs the table, sphere mesh
                         (vertex radius, closed surface, consistent winding)
   surface_points        centroid closed form
-Parity unpinned: project_rendering_blocks and render_shading (visual
-diagnostics outside the measured path) are checked only for basic invariants.
+  project_rendering_blocks  two blocks whose corners project to exact
+                        integers (power-of-two voxel size and focal length):
+                        rectangles, clamping at the border, min / max merge
+  render_shading        special geometries of the Phong terms (light and eye
+                        along the normal, normal perpendicular to or facing
+                        away from the light) and the background colour
 """
 import math
 import os
@@ -481,3 +485,60 @@ def test_pixel_march_f64_plane_and_pose_derivative():
     assert np.array_equal(sp, s0) and np.array_equal(sm, s0)
     dz = (vp[:, 2] - vm[:, 2]) / (2 * h)
     assert np.allclose(dz, -1.0, atol=1e-3)
+
+
+def test_rendering_blocks_closed_form():
+    """Voxel 1/64 m (block edge 0.125 m), camera at the origin looking down
+    +z, fx = fy = 48, c = (0, 0), an 8x8 z-range map (1/8 resolution). Block
+    (0, 0, 2) spans z 0.25..0.375 and projects to x, y in [0, 3] (corner
+    projections 0, 2, 3); block (-1, 0, 3) spans z 0.375..0.5 and projects to
+    x in [-2, 0] (clamped to [0, 0]) and y in [0, 2]; block (1, 1, 5) spans z
+    0.625..0.75 and its corners project to 1.0, 1.2, 2.0, 2.4 on each axis,
+    so floor / ceil give [1, 3]. Cells covered by several blocks take the
+    smallest near z and the largest far z; the rest keep 100 / 0."""
+    m = O.OracleMap(4096 + 512, 4096, 64, 1.0 / 64, 4.0 / 64, 0)
+    m.insert_block(0, 0, 2)
+    m.insert_block(-1, 0, 3)
+    m.insert_block(1, 1, 5)
+    m.insert_block(0, 0, -4)  # behind the camera: not visible
+    K = np.array([48.0, 48.0, 0.0, 0.0], np.float32)
+    eye = np.eye(4, dtype=np.float32).reshape(-1)
+    assert m.check_visible(64, 64, K, eye) == 3
+    n, zx, zy = m.project_rendering_blocks(8, 8, K, eye)
+    assert n == 3  # one 16x16 tile per block
+    ex = np.full((8, 8), 100.0, np.float32)
+    ey = np.zeros((8, 8), np.float32)
+    ex[0:4, 0:4] = 0.25
+    ey[0:4, 0:4] = 0.375
+    ex[0:3, 0] = 0.25   # min(0.25, 0.375)
+    ey[0:3, 0] = 0.5    # max(0.375, 0.5)
+    ey[1:4, 1:4] = 0.75  # block (1, 1, 5): max(0.375, 0.75); near z stays 0.25
+    assert np.array_equal(zx, ex), zx
+    assert np.array_equal(zy, ey), zy
+
+
+def test_render_shading_closed_form():
+    """Phong shading with ambient 0.3, diffuse 0.5, specular 0.2 (power 20),
+    white material, light at (5, 5, 5), eye at the origin. For a point on the
+    ray through (1, 1, 1) behind the origin, the light and eye directions are
+    both u = (1, 1, 1)/sqrt(3): normal u gives 0.3 + 0.5 + 0.2 = 1 (255);
+    a normal perpendicular to u gives ambient only (the mirror direction is
+    -u), 0.3 -> 76; normal -u gives ambient + specular, 0.5 -> 127. A NaN
+    vertex gets the background (4, 2, 2). One unit of rounding is allowed."""
+    u = np.ones(3) / np.sqrt(3.0)
+    p = np.array([-1.0, -1.0, -1.0])
+    perp = np.array([1.0, -1.0, 0.0]) / np.sqrt(2.0)
+    vmap = np.zeros((1, 4, 4), np.float32)
+    nmap = np.zeros((1, 4, 4), np.float32)
+    vmap[0, 0, :3] = np.nan
+    for i, n in enumerate((u, perp, -u)):
+        vmap[0, i + 1, :3] = p
+        vmap[0, i + 1, 3] = 1.0
+        nmap[0, i + 1, :3] = n
+        nmap[0, i + 1, 3] = 1.0
+    img = O.render_shading(vmap, nmap).astype(int)
+    assert img[0, 0].tolist() == [4, 2, 2, 255]
+    for i, want in enumerate((255, 76, 127)):
+        px = img[0, i + 1]
+        assert px[3] == 255 and px[0] == px[1] == px[2], px
+        assert abs(int(px[0]) - want) <= 1, (i, px, want)
