@@ -201,14 +201,18 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
  * (grad s . d) through the ray's origin and direction. Residuals with
  * |r| >= max_residual metres, misses and depth
  * outside [depth_min, depth_max] are left out. Each iteration is one launch
- * over all hypotheses; the 6x6 normal equations (Marquardt damping 1e-3) are
- * solved in double and T <- T exp(delta). out_poses (HOST, n_hyp * 16,
- * column-major) receives the refined poses; out_cost (HOST, n_hyp * (iters + 1)
- * * 2 floats) the {mean r^2, residual count} before each update and after
- * the last (mean -1 when no residual is left). out_system (HOST, may be NULL,
- * n_hyp * (iters + 1) * 27 doubles) receives the normal equations of each
- * evaluation: the 21 upper-triangle entries of J^T J row by row, then J^T r.
- * Synchronous. */
+ * over all hypotheses, then a Levenberg-Marquardt decision per hypothesis: a
+ * new pose is kept only if it does not raise the truncated cost (sum of r^2
+ * below max_residual, max_residual^2 for every other pixel with a valid
+ * depth, over the valid pixels), else the previous pose is restored and the
+ * damping (initially 1e-3 of diag(J^T J)) grows tenfold; accepted steps
+ * divide it by ten. The damped 6x6 normal equations are solved in double and
+ * T <- T exp(delta). out_poses (HOST, n_hyp * 16, column-major) receives the
+ * kept poses; out_cost (HOST, n_hyp * (iters + 1) * 2 floats) the {mean r^2,
+ * residual count} of the pose kept after each evaluation (mean -1 when no
+ * residual is left); out_system (HOST, may be NULL, n_hyp * (iters + 1) * 27
+ * doubles) that pose's normal equations: the 21 upper-triangle entries of
+ * J^T J row by row, then J^T r. Synchronous. */
 int vm_refine_poses(vm_map* map, const void* depth, int64_t depth_pitch, int32_t width, int32_t height,
                     int32_t depth_format, float depth_scale, const float K[4], const float* init_poses, int32_t n_hyp,
                     int32_t iters, int32_t stride, float max_residual, float* out_poses, float* out_cost,

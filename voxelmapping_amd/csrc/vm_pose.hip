@@ -14,10 +14,15 @@
 // oblique rays by a pose-dependent amount, which makes its exact derivative
 // (the raycast adjoint's) a poor Gauss-Newton model.
 //
-// Each iteration accumulates J^T J, J^T r, sum r^2 and the residual count per
-// hypothesis (per-workgroup partials, reduced in double), solves the damped
-// 6x6 normal equations by Cholesky in double and right-multiplies the pose by
-// exp(delta).
+// Each iteration accumulates J^T J, J^T r, sum r^2, the residual count and
+// the count of pixels with a valid depth per hypothesis (per-workgroup
+// partials, reduced in double), then takes a Levenberg-Marquardt step: the
+// pose is kept only if it lowers the truncated cost E = (sum r^2 over
+// residuals below max_residual + max_residual^2 per other valid pixel) /
+// valid pixels (else the previous pose and its normal equations come back and
+// the damping grows tenfold; an accepted step divides it by ten), and the
+// damped 6x6 normal equations are solved by Cholesky in double and the pose
+// right-multiplied by exp(delta).
 #include <algorithm>
 
 #include "vm_common.h"
@@ -29,7 +34,9 @@ namespace vmamd {
 
 namespace {
 
-constexpr int kGnTerms = 29;  // 21 of J^T J (upper triangle), 6 of J^T r, sum r^2, count
+// 21 of J^T J (upper triangle), 6 of J^T r, sum r^2, residual count, valid-depth count
+constexpr int kGnTerms = 30;
+constexpr int kLmState = kGnTerms + 2;  // accepted terms, their cost E, damping
 constexpr int kGnThreads = 256;
 
 __device__ inline float3 f3add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -131,6 +138,7 @@ __global__ __launch_bounds__(kGnThreads) void k_pose_gn(MapView m, DepthImage d,
         const int x = (i % sw) * stride, y = (i / sw) * stride;
         const float dep = load_depth<DF>(d, x, y);
         if (dep == dep && dep >= m.depthMin && dep <= m.depthMax) {
+            acc[29] = 1.f;
             RayReader<FMT> rd(m);
             float z, J[6];
             if (surface_depth(rd, x, y, K, poses[h], posesInv[h], vsInv, rtStep, z, J)) {
@@ -220,13 +228,17 @@ __device__ inline Affine rigid_inverse(const double T[16]) {
 // One workgroup of kSolveThreads per hypothesis: the partials are summed in
 // double by kSolveThreads / 32 groups of 32 threads (one term per thread,
 // every (kSolveThreads / 32)-th block per group) and the groups' sums added in
-// LDS; thread 0 then records the cost, solves
-// (J^T J + lambda diag(J^T J)) delta = -J^T r and sets T <- T exp(delta).
+// LDS; thread 0 then decides on the last step (Levenberg-Marquardt, above),
+// records the cost and normal equations of the pose kept, and unless this is
+// the final evaluation solves (J^T J + lambda diag(J^T J)) delta = -J^T r and
+// sets T <- T exp(delta). lm (kLmState doubles per hypothesis) and posesKept
+// (16 doubles) hold the accepted state between launches.
 constexpr int kSolveThreads = 256;
 __global__ __launch_bounds__(kSolveThreads) void k_pose_solve(const float* partials, int blocksPerHyp, int nHyp,
                                                               double* posesD, Affine* poses, Affine* posesInv,
-                                                              float* cost, double* system, int iter, int iters,
-                                                              int update, double lambda) {
+                                                              double* posesKept, double* lm, float* cost,
+                                                              double* system, int iter, int iters, int update,
+                                                              double lambda0, double maxRes) {
     constexpr int kGroups = kSolveThreads / 32;
     __shared__ double s_sum[kGroups][32];
     const int h = blockIdx.x;
@@ -244,6 +256,27 @@ __global__ __launch_bounds__(kSolveThreads) void k_pose_solve(const float* parti
         for (int g = 0; g < kGroups; ++g) t += s_sum[g][q];
         a[q] = t;
     }
+    double* st = lm + (int64_t)h * kLmState;
+    double* T = posesD + (int64_t)h * 16;
+    double* Tk = posesKept + (int64_t)h * 16;
+    const double nValid = a[29];
+    const double E = nValid > 0 ? (a[27] + (nValid - a[28]) * maxRes * maxRes) / nValid : 0.0;
+    double lambda;
+    if (iter == 0 || E <= st[kGnTerms]) {
+        // accept: this pose and its terms become the kept state
+        lambda = iter == 0 ? lambda0 : fmax(st[kGnTerms + 1] * 0.1, 1e-9);
+        for (int q = 0; q < kGnTerms; ++q) st[q] = a[q];
+        st[kGnTerms] = E;
+        for (int q = 0; q < 16; ++q) Tk[q] = T[q];
+    } else {
+        // reject: back to the kept pose, more damping
+        lambda = st[kGnTerms + 1] * 10.0;
+        for (int q = 0; q < kGnTerms; ++q) a[q] = st[q];
+        for (int q = 0; q < 16; ++q) T[q] = Tk[q];
+        poses[h] = affine_from(Tk);
+        posesInv[h] = rigid_inverse(Tk);
+    }
+    st[kGnTerms + 1] = lambda;
     const double n = a[28];
     for (int q = 0; q < 27; ++q) system[((int64_t)h * (iters + 1) + iter) * 27 + q] = a[q];
     cost[((int64_t)h * (iters + 1) + iter) * 2 + 0] = n > 0 ? (float)(a[27] / n) : -1.f;
@@ -280,13 +313,12 @@ __global__ __launch_bounds__(kSolveThreads) void k_pose_solve(const float* parti
         for (int q = i + 1; q < 6; ++q) s -= L[q * 6 + i] * delta[q];
         delta[i] = s / L[i * 6 + i];
     }
-    double E[16], Tn[16];
-    se3_exp(delta, E);
-    double* T = posesD + (int64_t)h * 16;
+    double Ex[16], Tn[16];
+    se3_exp(delta, Ex);
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) {
             double s = 0;
-            for (int q = 0; q < 4; ++q) s += T[i * 4 + q] * E[q * 4 + j];
+            for (int q = 0; q < 4; ++q) s += T[i * 4 + q] * Ex[q * 4 + j];
             Tn[i * 4 + j] = s;
         }
     for (int i = 0; i < 16; ++i) T[i] = Tn[i];
@@ -324,9 +356,9 @@ size_t refine_poses_scratch_bytes(int W, int H, int nHyp, int iters, int stride)
     const int sw = (W + stride - 1) / stride, sh = (H + stride - 1) / stride;
     const int bph = (sw * sh + kGnThreads - 1) / kGnThreads;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return up((size_t)nHyp * bph * kGnTerms * sizeof(float)) + up((size_t)nHyp * 16 * sizeof(double)) +
-           2 * up((size_t)nHyp * sizeof(Affine)) + up((size_t)nHyp * (iters + 1) * 2 * sizeof(float)) +
-           up((size_t)nHyp * (iters + 1) * 27 * sizeof(double));
+    return up((size_t)nHyp * bph * kGnTerms * sizeof(float)) + 2 * up((size_t)nHyp * 16 * sizeof(double)) +
+           up((size_t)nHyp * kLmState * sizeof(double)) + 2 * up((size_t)nHyp * sizeof(Affine)) +
+           up((size_t)nHyp * (iters + 1) * 2 * sizeof(float)) + up((size_t)nHyp * (iters + 1) * 27 * sizeof(double));
 }
 
 // posesIn: nHyp row-major 4x4 doubles (host). scratch: refine_poses_scratch_bytes.
@@ -344,6 +376,10 @@ hipError_t launch_refine_poses(const MapView& m, const DepthImage& d, const Intr
     p += up((size_t)nHyp * bph * kGnTerms * sizeof(float));
     double* posesD = (double*)p;
     p += up((size_t)nHyp * 16 * sizeof(double));
+    double* posesKept = (double*)p;
+    p += up((size_t)nHyp * 16 * sizeof(double));
+    double* lm = (double*)p;
+    p += up((size_t)nHyp * kLmState * sizeof(double));
     Affine* poses = (Affine*)p;
     p += up((size_t)nHyp * sizeof(Affine));
     Affine* posesInv = (Affine*)p;
@@ -363,7 +399,8 @@ hipError_t launch_refine_poses(const MapView& m, const DepthImage& d, const Intr
         }
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_pose_solve, dim3(nHyp), dim3(kSolveThreads), 0, s, partials, bph, nHyp, posesD, poses,
-                           posesInv, cost, system, it, iters, (int)(it < iters), lambda);
+                           posesInv, posesKept, lm, cost, system, it, iters, (int)(it < iters), lambda,
+                           (double)maxRes);
     }
     *posesOut = posesD;
     *costOut = cost;
