@@ -217,14 +217,22 @@ def pose_bench(m, w, depth, T, n_hyp: int = 32, iters: int = 8, stride: int = 4,
             "best_hypothesis_error_mm": round(1000 * float(np.linalg.norm(dT[:3, 3])), 2)}
 
 
-def scale_bench(frames: int = 24, warmup: int = 4) -> dict:
+def scale_bench(frames: int = 24, warmup: int = 4, track_every: int = 10, n_hyp: int = 32, iters: int = 8,
+                stride: int = 4, **synth_kw) -> dict:
     """The large-room workload at full size: 1280x720 uint16 depth fused at
     2 mm into 2^23 buckets / 2^22 float-TSDF blocks (24 GiB of voxels), fuse +
-    raycast + normals per frame."""
+    raycast + normals per frame. Every track_every-th frame, before it is
+    fused, n_hyp pose hypotheses around its true pose (U[-2, 2] cm,
+    U[-1.5, 1.5] deg) are refined against its depth by iters Gauss-Newton
+    iterations over every stride-th pixel (one launch per iteration for all
+    hypotheses); refinement time is reported apart from the fusion step.
+    synth_kw overrides workload fields (e.g. noise=0 for the geometry
+    without depth noise)."""
     import torch
     from voxelmapping_amd import computeNormal, synth
+    from voxelmapping_amd.autograd import se3_exp
     from voxelmapping_amd.mapping import DeviceMap
-    w = synth.room_large()
+    w = synth.room_large(**synth_kw)
     m = DeviceMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, w.voxel_format, 0.1, 5.0)
     n = warmup + frames
     deps = [torch.from_numpy(w.frame(i, want_rgb=False)[0].view(np.int16)).cuda().view(torch.uint16)
@@ -232,25 +240,45 @@ def scale_bench(frames: int = 24, warmup: int = 4) -> dict:
     poses = w.poses(n)
     vm = torch.zeros((w.height, w.width, 4), dtype=torch.float32, device="cuda")
     nm = torch.zeros_like(vm)
+    rng = np.random.default_rng(11)
+    refine_s, errs = [], []
 
     def step(i):
         m.fuse(deps[i], w.K, poses[i], depth_scale=w.depth_scale)
         m.render_scene(vm, w.K, poses[i])
         computeNormal(vm, nm)
 
+    def track(i):
+        T0 = poses[i].reshape(4, 4).T.astype(np.float64)
+        hyps = np.stack([T0 @ se3_exp(np.r_[rng.uniform(-1, 1, 3) * 0.02, np.radians(rng.uniform(-1, 1, 3) * 1.5)])
+                         for _ in range(n_hyp)]).astype(np.float32)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out, cost = m.refine_poses(deps[i], w.K, hyps, iters=iters, stride=stride, depth_scale=w.depth_scale)
+        refine_s.append(time.perf_counter() - t1)
+        best = int(np.argmin(cost[:, -1, 0] + (cost[:, -1, 0] < 0) * 1e9))
+        errs.append(float(np.linalg.norm((np.linalg.inv(T0) @ out[best].astype(np.float64))[:3, 3])))
+
     for i in range(warmup):
         step(i)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(warmup, n):
+        if i % track_every == 0:
+            track(i)
         step(i)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt = time.perf_counter() - t0 - sum(refine_s)
     st = m.stats()
     m.release()
     return {"workload": w.name, "frames": [warmup, n], "value": round(frames / dt, 2), "unit": "frames/s",
             "ms_per_step": round(1000 * dt / frames, 4), "allocated_blocks": st["allocated_blocks"],
-            "status": st["status"], "voxel_pool_gib": round(w.n_block * 512 * 12 / 2**30, 1)}
+            "status": st["status"], "voxel_pool_gib": round(w.n_block * 512 * 12 / 2**30, 1),
+            "tracking": {"every": track_every, "hypotheses": n_hyp, "iterations": iters,
+                         "pixels": ((w.width + stride - 1) // stride) * ((w.height + stride - 1) // stride),
+                         "refinements": len(refine_s),
+                         "ms_per_refinement": round(1000 * float(np.mean(refine_s)), 3) if refine_s else None,
+                         "best_hypothesis_error_mm": [round(1000 * e, 2) for e in errs]}}
 
 
 def adjoint_bench(reps: int = 20) -> dict:
@@ -424,6 +452,7 @@ def main():
         last = args.warmup + args.steps - 1
         out["pose_refine"] = pose_bench(m, w, depth_d[last].view(torch.uint16), w.poses(nframes)[last].reshape(4, 4).T)
         out["scale"] = scale_bench()
+        out["scale"]["tracking_without_depth_noise"] = scale_bench(noise=0.0, dropout=0.0)["tracking"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth.room_vga(), args.cpu_seconds)
     print(json.dumps(out), flush=True)
