@@ -159,8 +159,8 @@ def batched_bench(warmup: int, steps: int) -> dict:
             m.reset(stream)
         b.set_frame(0, stream)
         tune_steps = 1
-        best = b.tune(steps=tune_steps, stream=stream)  # 18 candidates x 1 step
-        done = 18 * tune_steps
+        best = b.tune(steps=tune_steps, stream=stream)  # 18 launch shapes + 3 scene-group counts, 1 step each
+        done = 21 * tune_steps
         if done > warmup:
             raise SystemExit("warmup must cover the autotuner's steps")
         for _ in range(warmup - done):
@@ -335,7 +335,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 16-scene batched comparison")
-    ap.add_argument("--batched-warmup", type=int, default=20)
+    ap.add_argument("--batched-warmup", type=int, default=24)
     ap.add_argument("--batched-steps", type=int, default=180)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (scales rocprofv3 byte counters)")

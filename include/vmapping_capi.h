@@ -271,6 +271,9 @@ typedef struct vm_batch_tuning {
     int32_t xcd_bands;            /* 1: give each XCD a contiguous range of ray tiles */
     int32_t visible_wg_per_cu;    /* visibility workgroups per compute unit */
     int32_t integrate_wg_per_cu;  /* integration workgroups (4 waves) per compute unit */
+    int32_t scene_groups;         /* 1..4: scenes dealt to this many groups (largest image first, to the
+                                     group with the fewest pixels), each group's kernels on a stream of
+                                     its own, forked from and joined back into the caller's stream */
 } vm_batch_tuning;
 
 int vm_batch_create(const vm_batch_scene* scenes, int32_t n_scenes, vm_batch** out_batch);
@@ -285,9 +288,11 @@ int vm_batch_capture(vm_batch* batch, void* stream);
 int vm_batch_get_tuning(vm_batch* batch, vm_batch_tuning* out_tuning);
 /* changes the launch shape; drops a captured graph */
 int vm_batch_set_tuning(vm_batch* batch, const vm_batch_tuning* tuning);
-/* Autotuner: times every candidate launch shape over `steps` steps each (the
- * steps fuse frames like any step) and applies the fastest; drops a captured
- * graph. */
+/* Autotuner: times the 18 launch shapes (ray tile 8x8 / 16x4 / 32x2, XCD
+ * bands on / off, 4 / 8 / 16 integration workgroups per CU) with one scene
+ * group, then 2, 3 and 4 scene groups with the fastest shape, `steps` steps
+ * each (the steps fuse frames like any step), and applies the fastest; drops
+ * a captured graph. */
 int vm_batch_tune(vm_batch* batch, void* stream, int32_t steps, vm_batch_tuning* out_best);
 
 #ifdef __cplusplus

@@ -119,6 +119,36 @@ def test_batch_graph_replay_and_tuning():
         same_images(b.vmaps[i], vm)
 
 
+def test_batch_scene_groups_on_streams():
+    """Scenes split over 2 and 3 groups, each group's kernels on a stream of
+    its own: eager steps and a captured graph with parallel branches still
+    equal the plain pipeline bit for bit, and the frame counter advances once
+    per step."""
+    import torch
+    ws = scenes_spec()
+    for groups in (2, 3):
+        b, maps = build_batch(ws, 8)
+        b.set_tuning(scene_groups=groups)
+        assert b.tuning()["scene_groups"] == groups
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            b.step(stream=s)
+            b.step(stream=s)
+            b.capture(stream=s)
+            for _ in range(3):
+                b.step(stream=s)
+                s.synchronize()
+                torch.cuda.synchronize()
+        s.synchronize()
+        assert b.frame(stream=s) == 5
+        for i, w in enumerate(ws):
+            m, vm, nm = plain_run(w, range(5))
+            same_maps(maps[i], m)
+            same_images(b.vmaps[i], vm)
+            same_images(b.nmaps[i][:-1, :-1], nm[:-1, :-1])
+        b.close()
+
+
 def test_batch_scene_against_oracle(oracle_threads):
     import torch
     ws = scenes_spec()[1:2]

@@ -99,6 +99,7 @@ class BatchTuning(C.Structure):
         ("xcd_bands", C.c_int32),
         ("visible_wg_per_cu", C.c_int32),
         ("integrate_wg_per_cu", C.c_int32),
+        ("scene_groups", C.c_int32),
     ]
 
 

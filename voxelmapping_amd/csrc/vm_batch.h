@@ -43,9 +43,14 @@ struct BatchTuning {
     int xcdBands;           // remap ray tiles into per-XCD contiguous bands
     int visibleWgPerCu;     // visibility workgroups per compute unit
     int integrateWgPerCu;   // integration workgroups (4 waves each) per CU
+    int sceneGroups;        // scenes split into this many groups, each on its own stream
 };
 
-hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, int df, bool color, hipStream_t s);
+// the five stage kernels of one step over the scenes of B; with advance, then
+// the frame counter increment
+hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, int df, bool color, hipStream_t s,
+                             bool advance = true);
+hipError_t batch_advance(int* frame, hipStream_t s);
 hipError_t batch_set_frame(int* frame, int v, hipStream_t s);
 int batch_max_scenes();
 

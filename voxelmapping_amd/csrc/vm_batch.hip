@@ -1,10 +1,13 @@
 // Multi-scene fuse + raycast: many independent maps of uneven image size,
 // voxel size and table size advanced in lockstep. One step runs five
-// kernels over the union of all scenes' work (allocation tiles, allocated
-// blocks, visible blocks, ray tiles, normal tiles), so small scenes share the
-// machine instead of each leaving most of it idle; the step can be captured
-// into a hipGraph and replayed, since every per-frame input (depth, colour,
-// pose) is found on the device from a device-side frame counter.
+// kernels over the union of a group of scenes' work (allocation tiles,
+// allocated blocks, visible blocks, ray tiles, normal tiles), so small scenes
+// share the machine instead of each leaving most of it idle; with several
+// groups, each group's kernels run on a stream of its own, so one group's
+// latency-bound stages overlap another's arithmetic-bound raycast. The step
+// can be captured into a hipGraph (the groups become parallel branches) and
+// replayed, since every per-frame input (depth, colour, pose) is found on the
+// device from a device-side frame counter.
 //
 // Per scene the arithmetic is exactly the single-map pipeline's
 // (vm_dev_fusion.h, vm_dev_ray.h), so batched and plain results are
@@ -169,7 +172,8 @@ __global__ void kb_set_frame(int* frame, int v) {
 
 }  // namespace
 
-hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, int df, bool color, hipStream_t s) {
+hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, int df, bool color, hipStream_t s,
+                             bool advance) {
     const int cu = num_compute_units();
     hipLaunchKernelGGL(kb_allocate, dim3(B.totalTile16), dim3(256), 0, s, B);
     hipLaunchKernelGGL(kb_visible, dim3(std::max(1, std::min(cu * t.visibleWgPerCu, B.maxChunks))), dim3(256), 0,
@@ -191,7 +195,12 @@ hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, 
     else if (fmt == kFmtRefRGB) hipLaunchKernelGGL(kb_raycast<kFmtRefRGB>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
     else hipLaunchKernelGGL(kb_raycast<kFmtF32>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
     hipLaunchKernelGGL(kb_normals, dim3(B.totalTile16), dim3(256), 0, s, B);
-    hipLaunchKernelGGL(kb_advance, dim3(1), dim3(64), 0, s, B.frame);
+    if (advance) hipLaunchKernelGGL(kb_advance, dim3(1), dim3(64), 0, s, B.frame);
+    return hipGetLastError();
+}
+
+hipError_t batch_advance(int* frame, hipStream_t s) {
+    hipLaunchKernelGGL(kb_advance, dim3(1), dim3(64), 0, s, frame);
     return hipGetLastError();
 }
 

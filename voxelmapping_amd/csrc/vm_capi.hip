@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -667,12 +668,15 @@ struct vm_batch {
     BatchScene* dScenes = nullptr;
     Affine* dPoses = nullptr;  // per scene: nFrames camToWorld, then nFrames worldToCam
     int* dFrame = nullptr;
-    BatchView view{};
-    BatchTuning tuning{8, 1, 4, 8};
+    std::vector<BatchView> groups;  // one view per scene group (contiguous in dScenes)
+    BatchTuning tuning{8, 1, 4, 8, 1};
     int fmt = 0, df = 0;
     bool color = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<hipStream_t> side;  // streams of groups 1.., created on first use
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
 };
 
 namespace {
@@ -684,32 +688,88 @@ void batch_drop_graph(vm_batch* b) {
     b->graph = nullptr;
 }
 
-// (re)build the per-scene device descriptors for the current tuning
+// (re)build the per-scene device descriptors for the current tuning. Scenes
+// are dealt to sceneGroups groups, largest image first, each to the group
+// with the fewest pixels so far; the device array holds the groups one after
+// the other, and each scene's tile offsets are relative to its group.
 int batch_upload(vm_batch* b) {
     const int n = (int)b->maps.size();
-    int tile16 = 0, ray = 0, chunks = 0, vis = 0;
-    const int tw = b->tuning.rayTileW, th = 64 / tw;
-    for (int i = 0; i < n; ++i) {
-        BatchScene& S = b->host[i];
-        const vm_batch_scene& d = b->desc[i];
-        S.tile16First = tile16;
-        S.tilesX16 = (d.width + 15) / 16;
-        tile16 += S.tilesX16 * ((d.height + 15) / 16);
-        S.rayTileFirst = ray;
-        ray += ((d.width + tw - 1) / tw) * ((d.height + th - 1) / th);
-        chunks += (S.m.nBlock + 255) / 256;
-        vis += S.m.nBlock < S.m.nEntry ? S.m.nBlock : S.m.nEntry;
+    const int G = std::max(1, std::min(b->tuning.sceneGroups, n));
+    std::vector<int> bySize(n);
+    for (int i = 0; i < n; ++i) bySize[i] = i;
+    std::stable_sort(bySize.begin(), bySize.end(), [&](int x, int y) {
+        return (int64_t)b->desc[x].width * b->desc[x].height > (int64_t)b->desc[y].width * b->desc[y].height;
+    });
+    std::vector<int> groupOf(n);
+    std::vector<int64_t> load(G, 0);
+    for (int i : bySize) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        groupOf[i] = g;
+        load[g] += (int64_t)b->desc[i].width * b->desc[i].height;
     }
-    b->view.scenes = b->dScenes;
-    b->view.nScenes = n;
-    b->view.frame = b->dFrame;
-    b->view.totalTile16 = tile16;
-    b->view.totalRayTiles = ray;
-    b->view.rayTilesPerXcd = (ray + 7) / 8;
-    b->view.maxChunks = chunks;
-    b->view.maxVisible = vis;
-    VM_HIP(hipMemcpy(b->dScenes, b->host.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice), "batch upload");
+    const int tw = b->tuning.rayTileW, th = 64 / tw;
+    std::vector<BatchScene> dev;
+    b->groups.clear();
+    for (int g = 0; g < G; ++g) {
+        BatchView v{};
+        v.scenes = b->dScenes + dev.size();
+        v.frame = b->dFrame;
+        int tile16 = 0, ray = 0, chunks = 0, vis = 0, cnt = 0;
+        for (int i = 0; i < n; ++i) {
+            if (groupOf[i] != g) continue;
+            BatchScene S = b->host[i];
+            const vm_batch_scene& d = b->desc[i];
+            S.tile16First = tile16;
+            S.tilesX16 = (d.width + 15) / 16;
+            tile16 += S.tilesX16 * ((d.height + 15) / 16);
+            S.rayTileFirst = ray;
+            ray += ((d.width + tw - 1) / tw) * ((d.height + th - 1) / th);
+            chunks += (S.m.nBlock + 255) / 256;
+            vis += S.m.nBlock < S.m.nEntry ? S.m.nBlock : S.m.nEntry;
+            dev.push_back(S);
+            ++cnt;
+        }
+        v.nScenes = cnt;
+        v.totalTile16 = tile16;
+        v.totalRayTiles = ray;
+        v.rayTilesPerXcd = (ray + 7) / 8;
+        v.maxChunks = chunks;
+        v.maxVisible = vis;
+        b->groups.push_back(v);
+    }
+    VM_HIP(hipMemcpy(b->dScenes, dev.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice), "batch upload");
     return VM_OK;
+}
+
+// one step: group 0 on s, group g on side stream g - 1, forked from and
+// joined back into s by events (parallel branches when captured), then the
+// frame counter increment on s
+hipError_t batch_launch_all(vm_batch* b, hipStream_t s) {
+    const int G = (int)b->groups.size();
+    if (G == 1) return batch_launch_step(b->groups[0], b->tuning, b->fmt, b->df, b->color, s);
+    hipError_t e = hipSuccess;
+    if (!b->fork) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    while (e == hipSuccess && (int)b->side.size() < G - 1) {
+        hipStream_t t = nullptr;
+        hipEvent_t ev = nullptr;
+        e = hipStreamCreateWithFlags(&t, hipStreamNonBlocking);
+        if (e == hipSuccess) b->side.push_back(t);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->join.push_back(ev);
+    }
+    if (e != hipSuccess) return e;
+    if ((e = hipEventRecord(b->fork, s)) != hipSuccess) return e;
+    for (int g = 1; g < G; ++g)
+        if ((e = hipStreamWaitEvent(b->side[g - 1], b->fork, 0)) != hipSuccess) return e;
+    for (int g = 0; g < G; ++g)
+        if ((e = batch_launch_step(b->groups[g], b->tuning, b->fmt, b->df, b->color, g ? b->side[g - 1] : s,
+                                   false)) != hipSuccess)
+            return e;
+    for (int g = 1; g < G; ++g) {
+        if ((e = hipEventRecord(b->join[g - 1], b->side[g - 1])) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s, b->join[g - 1], 0)) != hipSuccess) return e;
+    }
+    return batch_advance(b->dFrame, s);
 }
 
 void batch_touch_maps(vm_batch* b) {
@@ -825,6 +885,12 @@ int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
 int vm_batch_destroy(vm_batch* b) {
     if (!b) return VM_OK;
     batch_drop_graph(b);
+    for (hipStream_t t : b->side) {
+        hipStreamSynchronize(t);
+        hipStreamDestroy(t);
+    }
+    for (hipEvent_t ev : b->join) hipEventDestroy(ev);
+    if (b->fork) hipEventDestroy(b->fork);
     if (b->dScenes) hipFree(b->dScenes);
     if (b->dPoses) hipFree(b->dPoses);
     if (b->dFrame) hipFree(b->dFrame);
@@ -851,7 +917,7 @@ int vm_batch_step(vm_batch* b, void* stream) {
     if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
     hipStream_t s = (hipStream_t)stream;
     if (b->exec) VM_HIP(hipGraphLaunch(b->exec, s), "graph launch");
-    else VM_HIP(batch_launch_step(b->view, b->tuning, b->fmt, b->df, b->color, s), "batch step");
+    else VM_HIP(batch_launch_all(b, s), "batch step");
     batch_touch_maps(b);
     return VM_OK;
 }
@@ -862,7 +928,7 @@ int vm_batch_capture(vm_batch* b, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     batch_drop_graph(b);
     VM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-    hipError_t e = batch_launch_step(b->view, b->tuning, b->fmt, b->df, b->color, s);
+    hipError_t e = batch_launch_all(b, s);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &g);
     if (e != hipSuccess) {
@@ -881,6 +947,7 @@ int vm_batch_get_tuning(vm_batch* b, vm_batch_tuning* out) {
     out->xcd_bands = b->tuning.xcdBands;
     out->visible_wg_per_cu = b->tuning.visibleWgPerCu;
     out->integrate_wg_per_cu = b->tuning.integrateWgPerCu;
+    out->scene_groups = b->tuning.sceneGroups;
     return VM_OK;
 }
 
@@ -891,8 +958,11 @@ int vm_batch_set_tuning(vm_batch* b, const vm_batch_tuning* t) {
     if (t->visible_wg_per_cu < 1 || t->visible_wg_per_cu > 64 || t->integrate_wg_per_cu < 1 ||
         t->integrate_wg_per_cu > 64)
         return fail(VM_ERR_INVALID_ARG, "workgroups per CU must be in [1, 64]");
+    if (t->scene_groups < 1 || t->scene_groups > 4) return fail(VM_ERR_INVALID_ARG, "scene_groups must be in [1, 4]");
+    VM_HIP(hipDeviceSynchronize(), "synchronize");  // steps in flight read the scene array
     batch_drop_graph(b);
-    b->tuning = BatchTuning{t->ray_tile_width, t->xcd_bands ? 1 : 0, t->visible_wg_per_cu, t->integrate_wg_per_cu};
+    b->tuning = BatchTuning{t->ray_tile_width, t->xcd_bands ? 1 : 0, t->visible_wg_per_cu, t->integrate_wg_per_cu,
+                            t->scene_groups};
     return batch_upload(b);
 }
 
@@ -905,28 +975,36 @@ int vm_batch_tune(vm_batch* b, void* stream, int32_t steps, vm_batch_tuning* bes
     hipEvent_t e0, e1;
     VM_HIP(hipEventCreate(&e0), "event");
     VM_HIP(hipEventCreate(&e1), "event");
-    vm_batch_tuning best{8, 1, 4, 8};
+    vm_batch_tuning best{8, 1, 4, 8, 1};
     float bestMs = 1e30f;
     const int tws[] = {8, 16, 32};
     const int xcds[] = {1, 0};
     const int ints[] = {4, 8, 16};
     int rc = VM_OK;
+    // time `steps` steps of tuning t; keep it if fastest so far
+    auto trial = [&](const vm_batch_tuning& t) -> int {
+        int r;
+        if ((r = vm_batch_set_tuning(b, &t))) return r;
+        if (hipEventRecord(e0, s) != hipSuccess) return fail(VM_ERR_HIP, "event record");
+        for (int k = 0; k < steps; ++k)
+            if ((r = vm_batch_step(b, stream))) return r;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return fail(VM_ERR_HIP, "event timing");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < bestMs) { bestMs = ms; best = t; }
+        return VM_OK;
+    };
+    // the 18 launch shapes with one group, then 2..4 groups with the best shape
     for (int tw : tws)
         for (int xb : xcds)
-            for (int iw : ints) {
-                vm_batch_tuning t{tw, xb, 4, iw};
-                if ((rc = vm_batch_set_tuning(b, &t))) goto done;
-                if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(VM_ERR_HIP, "event record"); goto done; }
-                for (int k = 0; k < steps; ++k)
-                    if ((rc = vm_batch_step(b, stream))) goto done;
-                if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                    rc = fail(VM_ERR_HIP, "event timing");
-                    goto done;
-                }
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, e0, e1);
-                if (ms < bestMs) { bestMs = ms; best = t; }
-            }
+            for (int iw : ints)
+                if ((rc = trial(vm_batch_tuning{tw, xb, 4, iw, 1}))) goto done;
+    for (int g = 2; g <= 4 && g <= (int)b->maps.size(); ++g) {
+        vm_batch_tuning t = best;
+        t.scene_groups = g;
+        if ((rc = trial(t))) goto done;
+    }
     rc = vm_batch_set_tuning(b, &best);
     if (rc == VM_OK && best_out) *best_out = best;
 done:

@@ -404,12 +404,13 @@ class Batch:
             d.nmap, d.nmap_pitch = nm.data_ptr(), W * 16
         self._scenes = scenes
         h = C.c_void_p()
-        check(lib().vm_batch_create(descs, len(scenes), C.byref(h)))
+        self._lib = lib()  # kept for close() during interpreter shutdown
+        check(self._lib.vm_batch_create(descs, len(scenes), C.byref(h)))
         self._h = h
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().vm_batch_destroy(self._h)
+            self._lib.vm_batch_destroy(self._h)
             self._h = None
 
     __del__ = close
