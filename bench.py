@@ -281,6 +281,31 @@ def scale_bench(frames: int = 24, warmup: int = 4, track_every: int = 10, n_hyp:
                          "best_hypothesis_error_mm": [round(1000 * e, 2) for e in errs]}}
 
 
+def mesh_bench(m, reps: int = 5, cap: int = 20_000_000) -> dict:
+    """Marching cubes over every allocated block of the fused room_vga map
+    (vm_polygonize): mean time of reps calls, triangle count, and the
+    algorithmic bytes (8 B of TSDF + weight per voxel of each allocated
+    block read once, 72 B of vertices + face normals per triangle written)
+    over that time."""
+    import torch
+    verts = torch.empty((cap, 9), dtype=torch.float32, device="cuda")
+    norms = torch.empty_like(verts)
+    m.polygonize(verts, norms)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        _, tris = m.polygonize(verts, norms)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    A = m.stats()["allocated_blocks"]
+    byts = A * 512 * 8 + tris * 72
+    return {"workload": "room_vga map after the bench frames", "allocated_blocks": A, "triangles": tris,
+            "ms": round(ms, 3), "achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def adjoint_bench(reps: int = 20) -> dict:
     """Raycast forward vs its adjoint: 640x480 rays (fx = fy = 525) from a
     pose perturbed by (1 cm, 0.5 deg) into the fused room_small volume (50
@@ -445,6 +470,7 @@ def main():
         "allocated_blocks": s1["allocated_blocks"],
         "stage_avg_us": {n: round(v, 2) for n, v in stage_avg_us.items()},
         "status": s1["status"],
+        "mesh": mesh_bench(m),
     }
     if not args.no_batched:
         out["batched"] = batched_bench(args.batched_warmup, args.batched_steps)
