@@ -106,12 +106,47 @@ struct vm_map {
     // scratch of the raycast adjoint, grown on demand
     void* scratch = nullptr;
     size_t scratchBytes = 0;
+    // per-tile ray-distance bounds of the raycast (keys of 2 (tiles + 1)
+    // words), grown on demand; one epoch per raycast
+    unsigned long long* rayKeys = nullptr;
+    int rayKeysTiles = 0;
+    unsigned rayEpoch = 0;
 };
 
 namespace {
 
 int check_map(const vm_map* m) {
     if (!m) return fail(VM_ERR_INVALID_ARG, "map is NULL");
+    return VM_OK;
+}
+
+// The raycast's per-tile bounds for a W x H image with a fresh epoch. The
+// keys are (re)allocated only when the image has more tiles than any before
+// (room for 2^16 tiles at first) or when the epoch counter wraps; a fresh
+// buffer is filled on the stream (near keys all ones, far keys zero), which
+// reads as "no block" for every epoch.
+int ray_bounds_for(vm_map* m, int W, int H, hipStream_t s, RayBounds* out) {
+    const int n = ray_bounds_tiles(W, H);
+    if (n > m->rayKeysTiles || m->rayEpoch == 0xfffffffeu) {
+        const int cap = std::max(n, 1 << 16);
+        if (cap > m->rayKeysTiles) {
+            if (m->rayKeys) hipFree(m->rayKeys);
+            m->rayKeys = nullptr;
+            m->rayKeysTiles = 0;
+            hipError_t e = hipMalloc((void**)&m->rayKeys, sizeof(unsigned long long) * 2 * ((size_t)cap + 1));
+            if (e != hipSuccess) return hip_fail(e, "raycast bounds allocation");
+            m->rayKeysTiles = cap;
+        }
+        const int64_t words = 2 * ((int64_t)m->rayKeysTiles + 1);  // 32-bit words per half
+        VM_HIP(launch_fill_u32(m->rayKeys, 0xffffffffu, words, s), "raycast bounds fill");
+        VM_HIP(launch_fill_u32(m->rayKeys + m->rayKeysTiles + 1, 0u, words, s), "raycast bounds fill");
+        m->rayEpoch = 0;
+    }
+    m->rayEpoch++;
+    // the layout follows the capacity, not this image's tile count, so a key
+    // left by a launch at another image size sits in a slot of an older
+    // epoch, which every newer key replaces
+    *out = RayBounds{m->rayKeys, m->rayKeysTiles, m->rayEpoch};
     return VM_OK;
 }
 
@@ -229,7 +264,8 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
 int vm_map_destroy(vm_map* m) {
     if (!m) return VM_OK;
     MapView& v = m->view;
-    void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr, v.occ, m->scratch};
+    void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr, v.occ, m->scratch,
+                    m->rayKeys};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete m;
@@ -378,8 +414,10 @@ int vm_raycast(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H, cons
     if (int rc = check_pose_k(K, T)) return rc;
     float Ti[16];
     if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
+    RayBounds rb;
+    if (int rc = ray_bounds_for(m, W, H, (hipStream_t)stream, &rb)) return rc;
     VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, intrinsics_of(K), affine_of(T), affine_of(Ti), false,
-                          (hipStream_t)stream),
+                          (hipStream_t)stream, &rb),
            "raycast");
     return VM_OK;
 }
@@ -398,7 +436,9 @@ int vm_render_scene(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H,
         VM_HIP(launch_visible(m->view, W, H, k, affine_of(Ti), s), "visibility");
         remember_visibility(m, W, H, K, T);
     }
-    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s), "raycast");
+    RayBounds rb;
+    if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
+    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s, &rb), "raycast");
     return VM_OK;
 }
 

@@ -72,6 +72,15 @@ struct MapView {
     int fmt;
 };
 
+// Per-tile ray-distance bounds of one raycast (see vm_dev_ray.h): 2 (nTiles
+// + 1) epoch-stamped 64-bit keys; nTiles is the capacity, at least the
+// image's tile count.
+struct RayBounds {
+    unsigned long long* keys;
+    int nTiles;
+    unsigned epoch;
+};
+
 struct Intrinsics {
     float fx, fy, cx, cy;  // K(0,0), K(1,1), K(0,2), K(1,2)
     float ifx, ify;        // (float)(1.0 / K(0,0)): double divide, depth_fusion.cu:209

@@ -375,13 +375,41 @@ __device__ inline bool ray_surface(Ray& r, RayReader<FMT>& rd, const Affine& Tin
     return true;
 }
 
+// The loop's first samples of a ray while dist_s stays below nearLim, a
+// distance below which no sample can read a voxel of an allocated block:
+// each reads (1.0, invalid) and steps one voxel, so they run as the same
+// additions without reads. Samples k = 0..n-1 sit at dist_s + k < nearLim
+// and below distEnd - 2, where the loop condition holds.
+__device__ inline void ray_skip(Ray& r, float nearLim) {
+    const float q = fminf(nearLim, r.distEnd - 2.0f) - r.dist_s;
+    int n = q >= 1.0f ? (int)q : 0;
+    for (; n >= 2; n -= 2) {
+        r.res = make_float3(r.res.x + r.dir.x, r.res.y + r.dir.y, r.res.z + r.dir.z);
+        r.dist_s += 1.0f;
+        r.res = make_float3(r.res.x + r.dir.x, r.res.y + r.dir.y, r.res.z + r.dir.z);
+        r.dist_s += 1.0f;
+    }
+    if (n) {
+        r.res = make_float3(r.res.x + r.dir.x, r.res.y + r.dir.y, r.res.z + r.dir.z);
+        r.dist_s += 1.0f;
+    }
+}
+
 // MapRenderingDelegate::operator() for pixel (x, y); out points at the
 // pixel's float4. A miss writes only x = NaN, as the reference does.
+// [nearLim, farLim] (in the unit of dist_s) brackets every sample that can
+// read a voxel of an allocated block (ray_bounds, vm_raycast.hip): samples
+// before nearLim are skipped, and a ray that reaches farLim can only read
+// (1.0, invalid) from there to distEnd, which ends in a miss, so the march
+// stops there.
 template <int FMT>
 __device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y, const Intrinsics& K,
-                                     const Affine& pose, const Affine& Tinv, float vsInv, float rtStep) {
+                                     const Affine& pose, const Affine& Tinv, float vsInv, float rtStep,
+                                     float nearLim = 0.0f, float farLim = INFINITY) {
     Ray r;
     ray_begin(r, x, y, K, pose, vsInv);
+    r.distEnd = fminf(r.distEnd, farLim);
+    ray_skip(r, nearLim);
     RayReader<FMT> rd(m);
     int st;
     while ((st = ray_step(r, rd, rtStep)) == kRayGoing) {
@@ -389,6 +417,175 @@ __device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y,
     float4 v;
     if (st == kRaySurface && ray_surface(r, rd, Tinv, vsInv, rtStep, v)) *(float4*)out = v;
     else out[0] = __int_as_float(0x7fffffff);
+}
+
+// ---- per-tile ray-distance bounds ------------------------------------------
+//
+// For one raycast, per 8x8 pixel tile, the range of ray distance (voxels
+// along the ray, the unit of the march's dist_s) in which a ray of the tile
+// can read a voxel of an allocated block. Entries are 64-bit keys stamped
+// with the raycast's epoch, so a launch needs no clearing pass: near keys
+// (~epoch << 32 | near bits) take atomicMin, far keys (epoch << 32 | far
+// bits) atomicMax, and a key of another epoch reads as "no block" (near +inf,
+// far 0). Non-negative floats order as their bit patterns. An entry that
+// carries the current epoch from an earlier launch only widens the range.
+// Layout: near keys of tiles 0..n-1 and the all-tiles slot n, then the far
+// keys in the same order (2 (n + 1) words), n = RayBounds::nTiles (the
+// capacity, at least the image's tile count); struct RayBounds, vm_common.h.
+
+// Bound, in voxels, on how far the march's rounded positions and its dist_s
+// drift from the exact ray: at most dEnd additions (every step is >= 1
+// voxel) round a coordinate of magnitude <= cmax by <= ulp(cmax) / 2 each and
+// a step * dir product by <= 2^-24 * step; sqrt(3) for the three axes; 8 ulp
+// for the ray's start and direction; one voxel of slack. The longest ray,
+// |(ifx (x - cx), ify (y - cy), 1)| * 5 m / vs, is convex in (x, y), so its
+// maximum is at a corner pixel; the 1.001 factors cover the float rounding
+// of this estimate itself.
+__host__ __device__ inline float ray_drift_margin(int W, int H, const Intrinsics& K, float tx, float ty, float tz,
+                                                  float vsInv) {
+    float dEnd = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        const float ux = K.ifx * ((float)((k & 1) ? W - 1 : 0) - K.cx);
+        const float uy = K.ify * ((float)((k & 2) ? H - 1 : 0) - K.cy);
+        dEnd = fmaxf(dEnd, sqrtf(ux * ux + uy * uy + 1.0f) * 5.0f * vsInv);
+    }
+    dEnd = dEnd * 1.001f + 1.0f;
+    const float cmax = sqrtf(tx * tx + ty * ty + tz * tz) * vsInv * 1.001f + dEnd + 1.0f;
+    int e = 0;
+    frexpf(cmax, &e);  // cmax in [2^(e-1), 2^e)
+    const float ulp = ldexpf(1.0f, e - 24);
+    return 1.0f + 1.75f * dEnd * (0.5f * ulp + 0x1p-23f) + 8.0f * ulp;
+}
+
+// [nearLim, farLim] of tile t for raycast_pixel
+__device__ inline void ray_bounds_limits(const RayBounds& rb, int t, float margin, float& nearLim, float& farLim) {
+    const unsigned long long* k = rb.keys;
+    const int n1 = rb.nTiles + 1;
+    const unsigned long long nt = k[t], ng = k[rb.nTiles], ft = k[n1 + t], fg = k[n1 + rb.nTiles];
+    const unsigned e = rb.epoch, ne = ~rb.epoch;
+    float nr = INFINITY, fr = 0.0f;
+    if ((unsigned)(nt >> 32) == ne) nr = __uint_as_float((unsigned)nt);
+    if ((unsigned)(ng >> 32) == ne) nr = fminf(nr, __uint_as_float((unsigned)ng));
+    if ((unsigned)(ft >> 32) == e) fr = __uint_as_float((unsigned)ft);
+    if ((unsigned)(fg >> 32) == e) fr = fmaxf(fr, __uint_as_float((unsigned)fg));
+    nearLim = nr - margin;
+    farLim = fr + margin;
+}
+
+struct TileRect {
+    int tx0, ty0, w, n;  // first tile, width in tiles, tile count
+    unsigned nearBits, farBits;
+};
+
+constexpr int kBoundsChunk = 64;  // blocks per workgroup pass
+
+struct BoundsShared {
+    TileRect rects[kBoundsChunk];
+    int first[kBoundsChunk + 1];
+    int nRects;
+};
+
+// Bounds of the allocated blocks [base, base + 64) of map m, by a workgroup
+// of 256 threads. A block's box of march positions whose sample can read one
+// of its voxels is [8b - 1, 8b + 8) per axis (the -1 for the +1 corners of
+// the trilinear read), grown by `grow` voxels (ray_drift_margin). With all
+// corners in front of the camera the box projects into the hull of its
+// corners' projections, so every tile meeting that rectangle (plus one
+// pixel) gets the box's distance range from the camera centre c (voxel
+// units). A box reaching z < 1 cm goes into the all-tiles slot. Boxes wholly
+// behind z = 5 cm or beyond z = 5.05 m are out of every ray's reach (the
+// march samples z in [0.1, 5] m). The chunk's rectangles are appended to an
+// LDS list; a scan of their tile counts numbers every (rectangle, tile)
+// pair, and the 256 threads take the pairs in turn (a block near the camera
+// covers hundreds of tiles, one far away a few).
+__device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, int base, int A, int W, int H,
+                                        const Intrinsics& K, const Affine& Tinv, float3 c, float vs, float grow,
+                                        BoundsShared& sh) {
+    const int tid = threadIdx.x;
+    const int tilesX = (W + 7) >> 3;
+    const unsigned long long ne = (unsigned long long)(~rb.epoch) << 32, fe = (unsigned long long)rb.epoch << 32;
+    unsigned long long* nearK = rb.keys;
+    unsigned long long* farK = rb.keys + rb.nTiles + 1;
+    if (tid == 0) sh.nRects = 0;
+    __syncthreads();
+    const int i = base + tid;
+    if (tid < kBoundsChunk && i < A) {
+        const int4 b = m.blockPos[i];
+        if (b.x != kEmptyKey) {
+            const float lx = (float)(b.x * 8) - 1.0f - grow, hx = (float)(b.x * 8 + 8) + grow;
+            const float ly = (float)(b.y * 8) - 1.0f - grow, hy = (float)(b.y * 8 + 8) + grow;
+            const float lz = (float)(b.z * 8) - 1.0f - grow, hz = (float)(b.z * 8 + 8) + grow;
+            const float nx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.0f);
+            const float ny = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.0f);
+            const float nz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.0f);
+            const float ex = fmaxf(fabsf(lx - c.x), fabsf(hx - c.x));
+            const float ey = fmaxf(fabsf(ly - c.y), fabsf(hy - c.y));
+            const float ez = fmaxf(fabsf(lz - c.z), fabsf(hz - c.z));
+            const unsigned nb = __float_as_uint(sqrtf(nx * nx + ny * ny + nz * nz));
+            const unsigned fb = __float_as_uint(sqrtf(ex * ex + ey * ey + ez * ez));
+            float zmin = INFINITY, zmax = -INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY,
+                  vmax = -INFINITY;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float3 w =
+                    make_float3(((k & 1) ? hx : lx) * vs, ((k & 2) ? hy : ly) * vs, ((k & 4) ? hz : lz) * vs);
+                const float3 p = xform(Tinv, w);
+                zmin = fminf(zmin, p.z);
+                zmax = fmaxf(zmax, p.z);
+                const float u = K.fx * p.x / p.z + K.cx, v = K.fy * p.y / p.z + K.cy;
+                umin = fminf(umin, u); umax = fmaxf(umax, u);
+                vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+            }
+            if (zmax >= 0.05f && zmin <= 5.05f) {
+                if (zmin < 0.01f) {
+                    atomicMin(&nearK[rb.nTiles], ne | nb);
+                    atomicMax(&farK[rb.nTiles], fe | fb);
+                } else {
+                    const float x0 = fmaxf(floorf(umin) - 1.0f, 0.0f), x1 = fminf(ceilf(umax) + 1.0f, (float)(W - 1));
+                    const float y0 = fmaxf(floorf(vmin) - 1.0f, 0.0f), y1 = fminf(ceilf(vmax) + 1.0f, (float)(H - 1));
+                    if (x0 <= x1 && y0 <= y1) {
+                        const int tx0 = (int)x0 >> 3, tx1 = (int)x1 >> 3, ty0 = (int)y0 >> 3, ty1 = (int)y1 >> 3;
+                        const int slot = atomicAdd(&sh.nRects, 1);
+                        sh.rects[slot] = TileRect{tx0, ty0, tx1 - tx0 + 1, (tx1 - tx0 + 1) * (ty1 - ty0 + 1), nb, fb};
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int nr = sh.nRects;
+    // inclusive scan of the tile counts (Hillis-Steele over the chunk's slots)
+    int cnt = 0;
+    if (tid < kBoundsChunk) {
+        cnt = tid < nr ? sh.rects[tid].n : 0;
+        sh.first[tid + 1] = cnt;
+        if (tid == 0) sh.first[0] = 0;
+    }
+    __syncthreads();
+    for (int off = 1; off < kBoundsChunk; off <<= 1) {
+        const int add = (tid < kBoundsChunk && tid >= off) ? sh.first[tid + 1 - off] : 0;
+        __syncthreads();
+        if (tid < kBoundsChunk) {
+            cnt += add;
+            sh.first[tid + 1] = cnt;
+        }
+        __syncthreads();
+    }
+    const int total = sh.first[nr];
+    for (int k = tid; k < total; k += blockDim.x) {
+        int lo = 0, hi = nr - 1;  // last r with first[r] <= k
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sh.first[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        const TileRect& t = sh.rects[lo];
+        const int local = k - sh.first[lo];
+        const int ty = local / t.w;
+        const int tile = (t.ty0 + ty) * tilesX + t.tx0 + (local - ty * t.w);
+        atomicMin(&nearK[tile], ne | t.nearBits);
+        atomicMax(&farK[tile], fe | t.farBits);
+    }
+    __syncthreads();
 }
 
 // computeNormalKernel body, image_proc.cu:74-93, for pixel (x, y) with

@@ -26,9 +26,13 @@ hipError_t launch_visible(const MapView& m, int width, int height, const Intrins
                           const Affine& worldToCam, hipStream_t s);
 // MapRenderingDelegate, ray_tracing.cu:255-401. gate: skip when the visible
 // count is zero (the facade's gate, vmapping.cpp:87-88 / :101)
+// bounds: null, or the per-tile ray-distance bounds (RayBounds, vm_dev_ray.h)
+// for this image size and a fresh epoch, computed before the march
 hipError_t launch_raycast(const MapView& m, float* vmap, int64_t pitch, int width, int height,
                           const Intrinsics& K, const Affine& camToWorld, const Affine& worldToCam,
-                          bool gate, hipStream_t s);
+                          bool gate, hipStream_t s, const RayBounds* bounds = nullptr);
+// tiles of a width x height image (the bounds hold 2 (tiles + 1) 64-bit keys)
+inline int ray_bounds_tiles(int width, int height) { return ((width + 7) / 8) * ((height + 7) / 8); }
 // RenderingBlockDelegate + FillRenderingBlockFunctor, ray_tracing.cu:13-234
 hipError_t launch_rendering_blocks(const MapView& m, float* zx, float* zy, int64_t pitch,
                                    int cols8, int rows8, const Intrinsics& K,

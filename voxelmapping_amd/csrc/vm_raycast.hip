@@ -7,6 +7,9 @@
 // band traverse the same voxel blocks and share that XCD's L2.
 //
 // The per-pixel march is raycast_pixel (vm_dev_ray.h).
+#include <algorithm>
+#include <cmath>
+
 #include "vm_common.h"
 #include "vm_dev_ray.h"
 #include "vm_launch.h"
@@ -18,7 +21,8 @@ namespace {
 template <int FMT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_raycast(MapView m, float* vmap, int64_t pitch, int W, int H,
                                                 Intrinsics K, Affine pose, Affine Tinv, float vsInv, float rtStep,
-                                                int gate, int tilesX, int tilesPerXcd, int nTiles) {
+                                                int gate, int tilesX, int tilesPerXcd, int nTiles,
+                                                RayBounds rb, float margin) {
     const int g = blockIdx.x;
     const int t = (g & 7) * tilesPerXcd + (g >> 3);
     if (t >= nTiles) return;
@@ -26,8 +30,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int y = (t / tilesX) * 8 + (threadIdx.x >> 3);
     if (x >= W || y >= H) return;
     if (gate && m.ctr[kVisibleCount] == 0) return;
+    float nearLim = 0.0f, farLim = INFINITY;
+    if (rb.keys) ray_bounds_limits(rb, t, margin, nearLim, farLim);
     float* out = (float*)((uint8_t*)vmap + (int64_t)y * pitch) + x * 4;
-    raycast_pixel<FMT>(m, out, x, y, K, pose, Tinv, vsInv, rtStep);
+    raycast_pixel<FMT>(m, out, x, y, K, pose, Tinv, vsInv, rtStep, nearLim, farLim);
+}
+
+// Ray-distance bounds of every tile (ray_bounds_chunk), 64 allocated blocks
+// per workgroup pass.
+__global__ __launch_bounds__(256) void k_ray_bounds(MapView m, RayBounds rb, int W, int H, Intrinsics K, Affine Tinv,
+                                                    float3 c, float vs, float grow) {
+    __shared__ BoundsShared sh;
+    const int A = allocated_count(m.ctr, m.nBlock);
+    for (int base = blockIdx.x * kBoundsChunk; base < A; base += gridDim.x * kBoundsChunk)
+        ray_bounds_chunk(m, rb, base, A, W, H, K, Tinv, c, vs, grow, sh);
 }
 
 // RenderingBlockDelegate::createRenderingBlock (ray_tracing.cu:30-80) per
@@ -90,26 +106,44 @@ __global__ void k_zrange_init(float* zx, float* zy, int64_t pitch, int cols8, in
 
 }  // namespace
 
+hipError_t launch_ray_bounds(const MapView& m, const RayBounds& rb, int W, int H, const Intrinsics& K,
+                             const Affine& pose, const Affine& Tinv, float grow, hipStream_t s) {
+    const float vsInv = (float)(1.0 / (double)m.voxelSize);
+    const float3 c = make_float3(pose.t0 * vsInv, pose.t1 * vsInv, pose.t2 * vsInv);
+    const int need = (m.nBlock + kBoundsChunk - 1) / kBoundsChunk, g = num_compute_units() * 8;
+    hipLaunchKernelGGL(k_ray_bounds, dim3(need < g ? need : g), dim3(256), 0, s, m, rb, W, H, K, Tinv, c,
+                       m.voxelSize, grow);
+    return hipGetLastError();
+}
+
 hipError_t launch_raycast(const MapView& m, float* vmap, int64_t pitch, int W, int H, const Intrinsics& K,
-                          const Affine& pose, const Affine& Tinv, bool gate, hipStream_t s) {
+                          const Affine& pose, const Affine& Tinv, bool gate, hipStream_t s, const RayBounds* bounds) {
     const int tilesX = (W + 7) / 8, tilesY = (H + 7) / 8;
     const int nTiles = tilesX * tilesY;
     const int perXcd = (nTiles + 7) / 8;
     const dim3 grid(perXcd * 8);
     const float vsInv = (float)(1.0 / (double)m.voxelSize);  // ray_tracing.cu:430
     const float rtStep = m.trunc / m.voxelSize;               // :431
+    float margin = 0.0f;
+    RayBounds rb{nullptr, nTiles, 0u};
+    if (bounds) {
+        rb = *bounds;
+        margin = ray_drift_margin(W, H, K, pose.t0, pose.t1, pose.t2, vsInv);
+        hipError_t e = launch_ray_bounds(m, rb, W, H, K, pose, Tinv, margin, s);
+        if (e != hipSuccess) return e;
+    }
     switch (m.fmt) {
         case kFmtRef:
             hipLaunchKernelGGL(k_raycast<kFmtRef>, grid, dim3(64), 0, s, m, vmap, pitch, W, H, K, pose, Tinv,
-                               vsInv, rtStep, (int)gate, tilesX, perXcd, nTiles);
+                               vsInv, rtStep, (int)gate, tilesX, perXcd, nTiles, rb, margin);
             break;
         case kFmtRefRGB:
             hipLaunchKernelGGL(k_raycast<kFmtRefRGB>, grid, dim3(64), 0, s, m, vmap, pitch, W, H, K, pose, Tinv,
-                               vsInv, rtStep, (int)gate, tilesX, perXcd, nTiles);
+                               vsInv, rtStep, (int)gate, tilesX, perXcd, nTiles, rb, margin);
             break;
         default:
             hipLaunchKernelGGL(k_raycast<kFmtF32>, grid, dim3(64), 0, s, m, vmap, pitch, W, H, K, pose, Tinv,
-                               vsInv, rtStep, (int)gate, tilesX, perXcd, nTiles);
+                               vsInv, rtStep, (int)gate, tilesX, perXcd, nTiles, rb, margin);
     }
     return hipGetLastError();
 }
