@@ -24,6 +24,10 @@ struct BatchScene {
     int tile16First;          // first 16x16 tile of this scene in the flat tile space
     int tilesX16;
     int rayTileFirst;         // first ray tile of this scene
+    // per-tile ray-distance bounds (vm_dev_ray.h) of the scene's 8x8 pixel
+    // tiles; the epoch of a step is its frame index + 1
+    unsigned long long* rayKeys;
+    int rayTiles;
 };
 
 struct BatchView {
@@ -34,6 +38,7 @@ struct BatchView {
     int totalRayTiles;
     int rayTilesPerXcd;
     int maxChunks;             // sum over scenes of ceil(nBlock / 256)
+    int maxBoundsChunks;       // sum over scenes of ceil(nBlock / kBoundsChunk)
     int maxVisible;            // sum over scenes of min(nBlock, nEntry)
 };
 

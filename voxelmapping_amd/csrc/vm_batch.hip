@@ -1,7 +1,8 @@
 // Multi-scene fuse + raycast: many independent maps of uneven image size,
-// voxel size and table size advanced in lockstep. One step runs five
+// voxel size and table size advanced in lockstep. One step runs six
 // kernels over the union of a group of scenes' work (allocation tiles,
-// allocated blocks, visible blocks, ray tiles, normal tiles), so small scenes
+// allocated blocks, visible blocks, allocated blocks again for the ray
+// bounds, ray tiles, normal tiles), so small scenes
 // share the machine instead of each leaving most of it idle; with several
 // groups, each group's kernels run on a stream of its own, so one group's
 // latency-bound stages overlap another's arithmetic-bound raycast. The step
@@ -122,6 +123,35 @@ __global__ __launch_bounds__(256) void kb_integrate(BatchView B) {
     }
 }
 
+// ray-distance bounds of every scene's tiles: chunks of kBoundsChunk
+// allocated blocks of every scene, concatenated
+__global__ __launch_bounds__(256) void kb_ray_bounds(BatchView B) {
+    __shared__ BoundsShared sh;
+    __shared__ int s_first[kMaxScenes + 1];
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < B.nScenes; ++k) {
+            s_first[k] = acc;
+            acc += (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + kBoundsChunk - 1) / kBoundsChunk;
+        }
+        s_first[B.nScenes] = acc;
+    }
+    __syncthreads();
+    const int total = s_first[B.nScenes];
+    const int f = *B.frame;
+    for (int c = blockIdx.x; c < total; c += gridDim.x) {
+        const int s = scene_of(s_first, B.nScenes, c);
+        const BatchScene& S = B.scenes[s];
+        if (f >= S.nFrames) continue;
+        const Affine& T = S.poses[f];
+        const float grow = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
+        const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);
+        const RayBounds rb{S.rayKeys, S.rayTiles, (unsigned)f + 1u};
+        ray_bounds_chunk(S.m, rb, (c - s_first[s]) * kBoundsChunk, allocated_count(S.m.ctr, S.m.nBlock),
+                         S.d.width, S.d.height, S.K, S.posesInv[f], cc, S.m.voxelSize, grow, sh);
+    }
+}
+
 // ray tiles of every scene, concatenated; tile shape tw x (64 / tw); with
 // xcdBands the flat tile index is remapped so each XCD gets a contiguous range
 template <int FMT>
@@ -143,8 +173,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void kb
     const int y = (lt / tilesX) * th + (threadIdx.x / tw);
     if (x >= S.d.width || y >= S.d.height) return;
     if (S.m.ctr[kVisibleCount] == 0) return;  // the facade's gate (vmapping.cpp:87-88)
+    const Affine& T = S.poses[f];
+    const float margin = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
+    float nearLim, farLim;
+    ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, (unsigned)f + 1u}, (y >> 3) * ((S.d.width + 7) >> 3) + (x >> 3),
+                      margin, nearLim, farLim);
     float* out = (float*)((uint8_t*)S.vmap + (int64_t)y * S.vpitch) + x * 4;
-    raycast_pixel<FMT>(S.m, out, x, y, S.K, S.poses[f], S.posesInv[f], S.vsInv, S.rtStep);
+    raycast_pixel<FMT>(S.m, out, x, y, S.K, T, S.posesInv[f], S.vsInv, S.rtStep, nearLim, farLim);
 }
 
 __global__ __launch_bounds__(256) void kb_normals(BatchView B) {
@@ -190,6 +225,7 @@ hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, 
         else { if (color) VM_INTEGRATE(kFmtF32, kDepthF32, true); else VM_INTEGRATE(kFmtF32, kDepthF32, false); }
     }
 #undef VM_INTEGRATE
+    hipLaunchKernelGGL(kb_ray_bounds, dim3(std::max(1, std::min(cu * 8, B.maxBoundsChunks))), dim3(256), 0, s, B);
     const int tiles = t.xcdBands ? B.rayTilesPerXcd * 8 : B.totalRayTiles;
     if (fmt == kFmtRef) hipLaunchKernelGGL(kb_raycast<kFmtRef>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
     else if (fmt == kFmtRefRGB) hipLaunchKernelGGL(kb_raycast<kFmtRefRGB>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);

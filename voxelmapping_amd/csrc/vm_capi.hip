@@ -707,6 +707,7 @@ struct vm_batch {
     std::vector<BatchScene> host;
     BatchScene* dScenes = nullptr;
     Affine* dPoses = nullptr;  // per scene: nFrames camToWorld, then nFrames worldToCam
+    unsigned long long* dRayKeys = nullptr;  // per scene: 2 (tiles + 1) ray-bound keys
     int* dFrame = nullptr;
     std::vector<BatchView> groups;  // one view per scene group (contiguous in dScenes)
     BatchTuning tuning{8, 1, 4, 8, 1};
@@ -754,7 +755,7 @@ int batch_upload(vm_batch* b) {
         BatchView v{};
         v.scenes = b->dScenes + dev.size();
         v.frame = b->dFrame;
-        int tile16 = 0, ray = 0, chunks = 0, vis = 0, cnt = 0;
+        int tile16 = 0, ray = 0, chunks = 0, bchunks = 0, vis = 0, cnt = 0;
         for (int i = 0; i < n; ++i) {
             if (groupOf[i] != g) continue;
             BatchScene S = b->host[i];
@@ -765,6 +766,7 @@ int batch_upload(vm_batch* b) {
             S.rayTileFirst = ray;
             ray += ((d.width + tw - 1) / tw) * ((d.height + th - 1) / th);
             chunks += (S.m.nBlock + 255) / 256;
+            bchunks += (S.m.nBlock + kBoundsChunk - 1) / kBoundsChunk;
             vis += S.m.nBlock < S.m.nEntry ? S.m.nBlock : S.m.nEntry;
             dev.push_back(S);
             ++cnt;
@@ -774,6 +776,7 @@ int batch_upload(vm_batch* b) {
         v.totalRayTiles = ray;
         v.rayTilesPerXcd = (ray + 7) / 8;
         v.maxChunks = chunks;
+        v.maxBoundsChunks = bchunks;
         v.maxVisible = vis;
         b->groups.push_back(v);
     }
@@ -880,7 +883,23 @@ int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
         }
         off += 2 * (size_t)d.n_frames;
     }
+    // ray-bound keys: near halves all ones and far halves zero read as "no
+    // block" for every epoch (an epoch is a frame index + 1)
+    size_t nKeys = 0;
+    std::vector<size_t> keyOff(n);
+    for (int i = 0; i < n; ++i) {
+        keyOff[i] = nKeys;
+        nKeys += 2 * ((size_t)ray_bounds_tiles(scenes[i].width, scenes[i].height) + 1);
+    }
+    std::vector<unsigned long long> keys(nKeys, 0ull);
+    for (int i = 0; i < n; ++i) {
+        const size_t half = (size_t)ray_bounds_tiles(scenes[i].width, scenes[i].height) + 1;
+        std::fill(keys.begin() + keyOff[i], keys.begin() + keyOff[i] + half, ~0ull);
+    }
     hipError_t e = hipMalloc((void**)&b->dPoses, sizeof(Affine) * nPoses);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dRayKeys, sizeof(unsigned long long) * nKeys);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->dRayKeys, keys.data(), sizeof(unsigned long long) * nKeys, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dScenes, sizeof(BatchScene) * n);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dFrame, sizeof(int) * 64);
     if (e == hipSuccess) e = hipMemcpy(b->dPoses, poses.data(), sizeof(Affine) * nPoses, hipMemcpyHostToDevice);
@@ -911,6 +930,8 @@ int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
         S.npitch = d.nmap_pitch;
         S.vsInv = (float)(1.0 / (double)S.m.voxelSize);
         S.rtStep = S.m.trunc / S.m.voxelSize;
+        S.rayKeys = b->dRayKeys + keyOff[i];
+        S.rayTiles = ray_bounds_tiles(d.width, d.height);
         b->host.push_back(S);
         off += 2 * (size_t)d.n_frames;
     }
@@ -933,6 +954,7 @@ int vm_batch_destroy(vm_batch* b) {
     if (b->fork) hipEventDestroy(b->fork);
     if (b->dScenes) hipFree(b->dScenes);
     if (b->dPoses) hipFree(b->dPoses);
+    if (b->dRayKeys) hipFree(b->dRayKeys);
     if (b->dFrame) hipFree(b->dFrame);
     delete b;
     return VM_OK;

@@ -477,8 +477,6 @@ struct TileRect {
     unsigned nearBits, farBits;
 };
 
-constexpr int kBoundsChunk = 64;  // blocks per workgroup pass
-
 struct BoundsShared {
     TileRect rects[kBoundsChunk];
     int first[kBoundsChunk + 1];
