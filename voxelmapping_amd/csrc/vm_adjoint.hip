@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                                          Intrinsics K, Affine pose, Affine Tinv, float vsInv,
                                                          float rtStep, float* gTsdf, float* poseParts,
                                                          StackRec* stack, int* stats, int tilesX, int tilesPerXcd,
-                                                         int nTiles) {
+                                                         int nTiles, RayBounds rb, float margin) {
     __shared__ int s_keys[kAccSlots];
     __shared__ float s_vals[kAccSlots];
     const int lane = threadIdx.x;
@@ -129,7 +129,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             float3 bs = xform(pose, pn);
             bs = make_float3(bs.x * vsInv, bs.y * vsInv, bs.z * vsInv);
             const float3 pf = make_float3(ux * zFar, uy * zFar, zFar);
-            const float distEnd = norm3(pf) * vsInv;
+            // the ray-distance bounds of the tile end the march early as the
+            // forward raycast does (raycast_pixel); a miss has no adjoint
+            float nearLim, farLim;
+            ray_bounds_limits(rb, tile, margin, nearLim, farLim);
+            const float distEnd = fminf(norm3(pf) * vsInv, farLim);
             float3 be = xform(pose, pf);
             be = make_float3(be.x * vsInv, be.y * vsInv, be.z * vsInv);
             const float3 dvec = make_float3(be.x - bs.x, be.y - bs.y, be.z - bs.z);
@@ -152,6 +156,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             // the last pushed record stays in registers until the unit steps
             // after it are known (the next push, or the end of the march)
             StackRec pend{0.f, 0.f, 0.f, 0};
+            {
+                // unit steps below the near bound (ray_skip), counted as units
+                const float q = fminf(nearLim, distEnd - 2.0f) - dist_s;
+                for (int n = q >= 1.0f ? (int)q : 0; n > 0; --n) {
+                    res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+                    dist_s += 1.0f;
+                    ++units;
+                }
+            }
             while (dist_s < distEnd) {
                 last = sdf;
                 const int vx = floor_i(res.x), vy = floor_i(res.y), vz = floor_i(res.z);
@@ -363,7 +376,7 @@ size_t raycast_backward_scratch_bytes(int W, int H) {
 hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t vpitch, const float* gVertex,
                                    int64_t gvpitch, const float* gNormal, int64_t gnpitch, int W, int H,
                                    const Intrinsics& K, const Affine& pose, const Affine& Tinv, float* gTsdf,
-                                   double* gPose, void* scratch, int* stats, hipStream_t s) {
+                                   double* gPose, void* scratch, int* stats, const RayBounds& rb, hipStream_t s) {
     const int tilesX = (W + 7) / 8, nTiles = tilesX * ((H + 7) / 8);
     const int waves = (int)backward_waves(W, H);
     uint8_t* p = (uint8_t*)scratch;
@@ -377,8 +390,11 @@ hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t 
                        gvpitch, gNormal, gnpitch, gv, gpitch, W, H);
     const float vsInv = (float)(1.0 / (double)m.voxelSize);
     const float rtStep = m.trunc / m.voxelSize;
+    const float margin = ray_drift_margin(W, H, K, pose.t0, pose.t1, pose.t2, vsInv);
+    hipError_t e = launch_ray_bounds(m, rb, W, H, K, pose, Tinv, margin, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_raycast_backward, dim3((unsigned)waves), dim3(64), 0, s, m, gv, gpitch, W, H, K, pose, Tinv,
-                       vsInv, rtStep, gTsdf, parts, stack, stats, tilesX, waves / 8, nTiles);
+                       vsInv, rtStep, gTsdf, parts, stack, stats, tilesX, waves / 8, nTiles, rb, margin);
     hipLaunchKernelGGL(k_pose_reduce, dim3(6), dim3(256), 0, s, parts, waves, gPose);
     return hipGetLastError();
 }

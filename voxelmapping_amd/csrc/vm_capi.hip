@@ -527,8 +527,10 @@ int vm_raycast_backward(vm_map* m, const float* vmap, int64_t vpitch, const floa
     }
     int* stats = (int*)((uint8_t*)m->scratch + need - 256);
     VM_HIP(launch_fill_u32(stats, 0u, 2, s), "stats reset");
+    RayBounds rb;
+    if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
     VM_HIP(launch_raycast_backward(m->view, vmap, vpitch, gv, gvpitch, gn, gnpitch, W, H, intrinsics_of(K),
-                                   affine_of(T), affine_of(Ti), grad_tsdf, grad_pose, m->scratch, stats, s),
+                                   affine_of(T), affine_of(Ti), grad_tsdf, grad_pose, m->scratch, stats, rb, s),
            "raycast backward");
     if (out_stats) {
         int st[2];

@@ -55,7 +55,12 @@ size_t raycast_backward_scratch_bytes(int W, int H);
 hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t vpitch, const float* gVertex,
                                    int64_t gvpitch, const float* gNormal, int64_t gnpitch, int W, int H,
                                    const Intrinsics& K, const Affine& camToWorld, const Affine& worldToCam,
-                                   float* gTsdf, double* gPose, void* scratch, int* stats, hipStream_t s);
+                                   float* gTsdf, double* gPose, void* scratch, int* stats, const RayBounds& rb,
+                                   hipStream_t s);
+// the per-tile ray-distance bounds of rb's epoch for a raycast at this pose;
+// grow = ray_drift_margin of the pose (vm_dev_ray.h)
+hipError_t launch_ray_bounds(const MapView& m, const RayBounds& rb, int W, int H, const Intrinsics& K,
+                             const Affine& camToWorld, const Affine& worldToCam, float grow, hipStream_t s);
 
 // device-side fill of 32-bit words (graph-capturable replacement of memset)
 hipError_t launch_fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t s);
