@@ -32,6 +32,26 @@ namespace {
 
 constexpr int kMaxScenes = 64;
 
+// s_first[k] = sum of count(j) over scenes j < k, s_first[n] = the total
+// (n >= 1), by a scan in the workgroup's first wave: lane j reads scene j's
+// count, so the scenes' counters are read in one memory round trip
+template <class F>
+__device__ inline void scene_prefix(int n, int* s_first, F count) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const int v = lane < n ? count(lane) : 0;
+        int incl = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+        }
+        if (lane < n) s_first[lane] = incl - v;
+        if (lane == n - 1) s_first[n] = incl;
+    }
+    __syncthreads();
+}
+
 // scene of a flat work index: the last scene whose first index is <= i
 __device__ inline int scene_of(const int* first, int n, int i) {
     int s = 0;
@@ -65,15 +85,8 @@ __global__ __launch_bounds__(256) void kb_allocate(BatchView B) {
 __global__ __launch_bounds__(256) void kb_visible(BatchView B) {
     __shared__ VisibleShared sh;
     __shared__ int s_first[kMaxScenes + 1];
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < B.nScenes; ++k) {
-            s_first[k] = acc;
-            acc += (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + 255) / 256;
-        }
-        s_first[B.nScenes] = acc;
-    }
-    __syncthreads();
+    scene_prefix(B.nScenes, s_first,
+                 [&](int k) { return (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + 255) / 256; });
     const int total = s_first[B.nScenes];
     const int f = *B.frame;
     for (int c = blockIdx.x; c < total; c += gridDim.x) {
@@ -92,15 +105,7 @@ __global__ __launch_bounds__(256) void kb_visible(BatchView B) {
 template <int FMT, int DF, bool COLOR>
 __global__ __launch_bounds__(256) void kb_integrate(BatchView B) {
     __shared__ int s_first[kMaxScenes + 1];
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < B.nScenes; ++k) {
-            s_first[k] = acc;
-            acc += B.scenes[k].m.ctr[kVisibleCount];
-        }
-        s_first[B.nScenes] = acc;
-    }
-    __syncthreads();
+    scene_prefix(B.nScenes, s_first, [&](int k) { return B.scenes[k].m.ctr[kVisibleCount]; });
     const int total = s_first[B.nScenes];
     const int f = *B.frame;
     const int lane = threadIdx.x & 63;
@@ -128,15 +133,9 @@ __global__ __launch_bounds__(256) void kb_integrate(BatchView B) {
 __global__ __launch_bounds__(256) void kb_ray_bounds(BatchView B) {
     __shared__ BoundsShared sh;
     __shared__ int s_first[kMaxScenes + 1];
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < B.nScenes; ++k) {
-            s_first[k] = acc;
-            acc += (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + kBoundsChunk - 1) / kBoundsChunk;
-        }
-        s_first[B.nScenes] = acc;
-    }
-    __syncthreads();
+    scene_prefix(B.nScenes, s_first, [&](int k) {
+        return (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + kBoundsChunk - 1) / kBoundsChunk;
+    });
     const int total = s_first[B.nScenes];
     const int f = *B.frame;
     for (int c = blockIdx.x; c < total; c += gridDim.x) {
