@@ -128,13 +128,14 @@ __global__ __launch_bounds__(256) void kb_integrate(BatchView B) {
     }
 }
 
-// ray-distance bounds of every scene's tiles: chunks of kBoundsChunk
+// ray-distance bounds of every scene's tiles: chunks of kBatchBoundsChunk
 // allocated blocks of every scene, concatenated
-__global__ __launch_bounds__(256) void kb_ray_bounds(BatchView B) {
-    __shared__ BoundsShared sh;
+__global__ __launch_bounds__(kBatchBoundsChunk) void kb_ray_bounds(BatchView B) {
+    __shared__ BoundsShared<kBatchBoundsChunk> sh;
     __shared__ int s_first[kMaxScenes + 1];
     scene_prefix(B.nScenes, s_first, [&](int k) {
-        return (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + kBoundsChunk - 1) / kBoundsChunk;
+        return (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + kBatchBoundsChunk - 1) /
+               kBatchBoundsChunk;
     });
     const int total = s_first[B.nScenes];
     const int f = *B.frame;
@@ -146,8 +147,9 @@ __global__ __launch_bounds__(256) void kb_ray_bounds(BatchView B) {
         const float grow = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
         const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);
         const RayBounds rb{S.rayKeys, S.rayTiles, (unsigned)f + 1u};
-        ray_bounds_chunk(S.m, rb, (c - s_first[s]) * kBoundsChunk, allocated_count(S.m.ctr, S.m.nBlock),
-                         S.d.width, S.d.height, S.K, S.posesInv[f], cc, S.m.voxelSize, grow, sh);
+        ray_bounds_chunk<kBatchBoundsChunk>(S.m, rb, (c - s_first[s]) * kBatchBoundsChunk,
+                                            allocated_count(S.m.ctr, S.m.nBlock), S.d.width, S.d.height, S.K,
+                                            S.posesInv[f], cc, S.m.voxelSize, grow, sh);
     }
 }
 
@@ -224,7 +226,8 @@ hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, 
         else { if (color) VM_INTEGRATE(kFmtF32, kDepthF32, true); else VM_INTEGRATE(kFmtF32, kDepthF32, false); }
     }
 #undef VM_INTEGRATE
-    hipLaunchKernelGGL(kb_ray_bounds, dim3(std::max(1, std::min(cu * 8, B.maxBoundsChunks))), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(kb_ray_bounds, dim3(std::max(1, std::min(cu * 8, B.maxBoundsChunks))), dim3(kBatchBoundsChunk),
+                       0, s, B);
     const int tiles = t.xcdBands ? B.rayTilesPerXcd * 8 : B.totalRayTiles;
     if (fmt == kFmtRef) hipLaunchKernelGGL(kb_raycast<kFmtRef>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
     else if (fmt == kFmtRefRGB) hipLaunchKernelGGL(kb_raycast<kFmtRefRGB>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);

@@ -246,7 +246,11 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
             return fail(VM_ERR_OUT_OF_MEMORY, msg);
         }
     }
-    int rc = vm_map_reset(m, nullptr);
+    // the raycast's tile bounds for images up to 2^16 8x8 tiles, so that a
+    // raycast of such an image allocates nothing
+    RayBounds rb0;
+    int rc = ray_bounds_for(m, 1, 1, nullptr, &rb0);
+    if (rc == VM_OK) rc = vm_map_reset(m, nullptr);
     if (rc != VM_OK) {
         std::string msg = g_error;
         vm_map_destroy(m);
@@ -768,7 +772,7 @@ int batch_upload(vm_batch* b) {
             S.rayTileFirst = ray;
             ray += ((d.width + tw - 1) / tw) * ((d.height + th - 1) / th);
             chunks += (S.m.nBlock + 255) / 256;
-            bchunks += (S.m.nBlock + kBoundsChunk - 1) / kBoundsChunk;
+            bchunks += (S.m.nBlock + kBatchBoundsChunk - 1) / kBatchBoundsChunk;
             vis += S.m.nBlock < S.m.nEntry ? S.m.nBlock : S.m.nEntry;
             dev.push_back(S);
             ++cnt;

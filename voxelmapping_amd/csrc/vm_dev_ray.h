@@ -477,14 +477,18 @@ struct TileRect {
     unsigned nearBits, farBits;
 };
 
+// A rectangle of at most this many tiles is written by the thread that found
+// it; larger ones (blocks near the camera) are shared by the workgroup.
+constexpr int kBoundsLightTiles = 32;
+
+template <int CH>
 struct BoundsShared {
-    TileRect rects[kBoundsChunk];
-    int first[kBoundsChunk + 1];
-    int nRects;
+    TileRect heavy[CH];
+    int nHeavy;
 };
 
-// Bounds of the allocated blocks [base, base + 64) of map m, by a workgroup
-// of 256 threads. A block's box of march positions whose sample can read one
+// Bounds of the allocated blocks [base, base + CH) of map m, by a workgroup
+// of CH threads. A block's box of march positions whose sample can read one
 // of its voxels is [8b - 1, 8b + 8) per axis (the -1 for the +1 corners of
 // the trilinear read), grown by `grow` voxels (ray_drift_margin). With all
 // corners in front of the camera the box projects into the hull of its
@@ -492,27 +496,28 @@ struct BoundsShared {
 // pixel) gets the box's distance range from the camera centre c (voxel
 // units). A box reaching z < 1 cm goes into the all-tiles slot. Boxes wholly
 // behind z = 5 cm or beyond z = 5.05 m are out of every ray's reach (the
-// march samples z in [0.1, 5] m). The chunk's rectangles are appended to an
-// LDS list; a scan of their tile counts numbers every (rectangle, tile)
-// pair, and the 256 threads take the pairs in turn (a block near the camera
-// covers hundreds of tiles, one far away a few).
+// march samples z in [0.1, 5] m). Most rectangles are a few tiles and their
+// thread writes them; the few larger ones are appended to an LDS list and
+// the workgroup's threads take their tiles in turn.
+template <int CH>
 __device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, int base, int A, int W, int H,
                                         const Intrinsics& K, const Affine& Tinv, float3 c, float vs, float grow,
-                                        BoundsShared& sh) {
+                                        BoundsShared<CH>& sh) {
     const int tid = threadIdx.x;
     const int tilesX = (W + 7) >> 3;
     const unsigned long long ne = (unsigned long long)(~rb.epoch) << 32, fe = (unsigned long long)rb.epoch << 32;
     unsigned long long* nearK = rb.keys;
     unsigned long long* farK = rb.keys + rb.nTiles + 1;
-    if (tid == 0) sh.nRects = 0;
+    if (tid == 0) sh.nHeavy = 0;
     __syncthreads();
     const int i = base + tid;
-    if (tid < kBoundsChunk && i < A) {
+    if (i < A) {
         const int4 b = m.blockPos[i];
         if (b.x != kEmptyKey) {
             const float lx = (float)(b.x * 8) - 1.0f - grow, hx = (float)(b.x * 8 + 8) + grow;
             const float ly = (float)(b.y * 8) - 1.0f - grow, hy = (float)(b.y * 8 + 8) + grow;
             const float lz = (float)(b.z * 8) - 1.0f - grow, hz = (float)(b.z * 8 + 8) + grow;
+            // distance range of the box from the camera centre
             const float nx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.0f);
             const float ny = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.0f);
             const float nz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.0f);
@@ -543,45 +548,31 @@ __device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, i
                     const float y0 = fmaxf(floorf(vmin) - 1.0f, 0.0f), y1 = fminf(ceilf(vmax) + 1.0f, (float)(H - 1));
                     if (x0 <= x1 && y0 <= y1) {
                         const int tx0 = (int)x0 >> 3, tx1 = (int)x1 >> 3, ty0 = (int)y0 >> 3, ty1 = (int)y1 >> 3;
-                        const int slot = atomicAdd(&sh.nRects, 1);
-                        sh.rects[slot] = TileRect{tx0, ty0, tx1 - tx0 + 1, (tx1 - tx0 + 1) * (ty1 - ty0 + 1), nb, fb};
+                        const int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+                        if (n <= kBoundsLightTiles) {
+                            for (int ty = ty0; ty <= ty1; ++ty)
+                                for (int tx = tx0; tx <= tx1; ++tx) {
+                                    atomicMin(&nearK[ty * tilesX + tx], ne | nb);
+                                    atomicMax(&farK[ty * tilesX + tx], fe | fb);
+                                }
+                        } else {
+                            sh.heavy[atomicAdd(&sh.nHeavy, 1)] = TileRect{tx0, ty0, tx1 - tx0 + 1, n, nb, fb};
+                        }
                     }
                 }
             }
         }
     }
     __syncthreads();
-    const int nr = sh.nRects;
-    // inclusive scan of the tile counts (Hillis-Steele over the chunk's slots)
-    int cnt = 0;
-    if (tid < kBoundsChunk) {
-        cnt = tid < nr ? sh.rects[tid].n : 0;
-        sh.first[tid + 1] = cnt;
-        if (tid == 0) sh.first[0] = 0;
-    }
-    __syncthreads();
-    for (int off = 1; off < kBoundsChunk; off <<= 1) {
-        const int add = (tid < kBoundsChunk && tid >= off) ? sh.first[tid + 1 - off] : 0;
-        __syncthreads();
-        if (tid < kBoundsChunk) {
-            cnt += add;
-            sh.first[tid + 1] = cnt;
+    const int nh = sh.nHeavy;
+    for (int r = 0; r < nh; ++r) {
+        const TileRect t = sh.heavy[r];
+        for (int k = tid; k < t.n; k += CH) {
+            const int ty = k / t.w;
+            const int tile = (t.ty0 + ty) * tilesX + t.tx0 + (k - ty * t.w);
+            atomicMin(&nearK[tile], ne | t.nearBits);
+            atomicMax(&farK[tile], fe | t.farBits);
         }
-        __syncthreads();
-    }
-    const int total = sh.first[nr];
-    for (int k = tid; k < total; k += blockDim.x) {
-        int lo = 0, hi = nr - 1;  // last r with first[r] <= k
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (sh.first[mid] <= k) lo = mid; else hi = mid - 1;
-        }
-        const TileRect& t = sh.rects[lo];
-        const int local = k - sh.first[lo];
-        const int ty = local / t.w;
-        const int tile = (t.ty0 + ty) * tilesX + t.tx0 + (local - ty * t.w);
-        atomicMin(&nearK[tile], ne | t.nearBits);
-        atomicMax(&farK[tile], fe | t.farBits);
     }
     __syncthreads();
 }
