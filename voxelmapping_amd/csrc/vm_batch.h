@@ -38,7 +38,6 @@ struct BatchView {
     int totalRayTiles;
     int rayTilesPerXcd;
     int maxChunks;             // sum over scenes of ceil(nBlock / 256)
-    int maxBoundsChunks;       // sum over scenes of ceil(nBlock / kBatchBoundsChunk)
     int maxVisible;            // sum over scenes of min(nBlock, nEntry)
 };
 

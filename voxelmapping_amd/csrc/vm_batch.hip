@@ -1,8 +1,8 @@
 // Multi-scene fuse + raycast: many independent maps of uneven image size,
-// voxel size and table size advanced in lockstep. One step runs six
+// voxel size and table size advanced in lockstep. One step runs five
 // kernels over the union of a group of scenes' work (allocation tiles,
-// allocated blocks, visible blocks, allocated blocks again for the ray
-// bounds, ray tiles, normal tiles), so small scenes
+// allocated blocks -- visibility and the raycast's ray-distance bounds --,
+// visible blocks, ray tiles, normal tiles), so small scenes
 // share the machine instead of each leaving most of it idle; with several
 // groups, each group's kernels run on a stream of its own, so one group's
 // latency-bound stages overlap another's arithmetic-bound raycast. The step
@@ -81,9 +81,11 @@ __global__ __launch_bounds__(256) void kb_allocate(BatchView B) {
     else allocate_pixel<kDepthF32>(S.m, d, S.K, S.poses[f], x, y);
 }
 
-// chunks of 256 allocated blocks of every scene, concatenated
+// chunks of 256 allocated blocks of every scene, concatenated: the visible
+// blocks, and the ray-distance bounds of the step's raycast (same pose)
 __global__ __launch_bounds__(256) void kb_visible(BatchView B) {
     __shared__ VisibleShared sh;
+    __shared__ BoundsShared<256> bsh;
     __shared__ int s_first[kMaxScenes + 1];
     scene_prefix(B.nScenes, s_first,
                  [&](int k) { return (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + 255) / 256; });
@@ -96,8 +98,16 @@ __global__ __launch_bounds__(256) void kb_visible(BatchView B) {
         const int A = allocated_count(S.m.ctr, S.m.nBlock);
         const DepthImage d{};
         const ColorImage col{};
+        const int base = (c - s_first[s]) * 256;
         visible_chunk<false, kFmtRef, kDepthF32, false>(S.m, d, col, S.K, S.posesInv[f], S.d.width, S.d.height,
-                                                         (c - s_first[s]) * 256, A, sh);
+                                                         base, A, sh);
+        const Affine& T = S.poses[f];
+        const float grow = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
+        const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);
+        const int i = base + (int)threadIdx.x;
+        const int4 b = i < A ? S.m.blockPos[i] : make_int4(kEmptyKey, 0, 0, 0);
+        ray_bounds_blocks<256>(b, RayBounds{S.rayKeys, S.rayTiles, (unsigned)f + 1u}, S.d.width, S.d.height, S.K,
+                               S.posesInv[f], cc, S.m.voxelSize, grow, bsh);
     }
 }
 
@@ -125,31 +135,6 @@ __global__ __launch_bounds__(256) void kb_integrate(BatchView B) {
         const HashEntry e = S.m.visible[i - s_first[s]];
         integrate_block<FMT, DF, COLOR>(S.m, d, col, S.K, S.posesInv[f], e.ptr / kBlockVox,
                                         make_int4(e.x, e.y, e.z, 0), lane);
-    }
-}
-
-// ray-distance bounds of every scene's tiles: chunks of kBatchBoundsChunk
-// allocated blocks of every scene, concatenated
-__global__ __launch_bounds__(kBatchBoundsChunk) void kb_ray_bounds(BatchView B) {
-    __shared__ BoundsShared<kBatchBoundsChunk> sh;
-    __shared__ int s_first[kMaxScenes + 1];
-    scene_prefix(B.nScenes, s_first, [&](int k) {
-        return (allocated_count(B.scenes[k].m.ctr, B.scenes[k].m.nBlock) + kBatchBoundsChunk - 1) /
-               kBatchBoundsChunk;
-    });
-    const int total = s_first[B.nScenes];
-    const int f = *B.frame;
-    for (int c = blockIdx.x; c < total; c += gridDim.x) {
-        const int s = scene_of(s_first, B.nScenes, c);
-        const BatchScene& S = B.scenes[s];
-        if (f >= S.nFrames) continue;
-        const Affine& T = S.poses[f];
-        const float grow = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
-        const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);
-        const RayBounds rb{S.rayKeys, S.rayTiles, (unsigned)f + 1u};
-        ray_bounds_chunk<kBatchBoundsChunk>(S.m, rb, (c - s_first[s]) * kBatchBoundsChunk,
-                                            allocated_count(S.m.ctr, S.m.nBlock), S.d.width, S.d.height, S.K,
-                                            S.posesInv[f], cc, S.m.voxelSize, grow, sh);
     }
 }
 
@@ -226,8 +211,6 @@ hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, 
         else { if (color) VM_INTEGRATE(kFmtF32, kDepthF32, true); else VM_INTEGRATE(kFmtF32, kDepthF32, false); }
     }
 #undef VM_INTEGRATE
-    hipLaunchKernelGGL(kb_ray_bounds, dim3(std::max(1, std::min(cu * 8, B.maxBoundsChunks))), dim3(kBatchBoundsChunk),
-                       0, s, B);
     const int tiles = t.xcdBands ? B.rayTilesPerXcd * 8 : B.totalRayTiles;
     if (fmt == kFmtRef) hipLaunchKernelGGL(kb_raycast<kFmtRef>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);
     else if (fmt == kFmtRefRGB) hipLaunchKernelGGL(kb_raycast<kFmtRefRGB>, dim3(tiles), dim3(64), 0, s, B, t.rayTileW, t.xcdBands);

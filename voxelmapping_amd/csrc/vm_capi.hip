@@ -761,7 +761,7 @@ int batch_upload(vm_batch* b) {
         BatchView v{};
         v.scenes = b->dScenes + dev.size();
         v.frame = b->dFrame;
-        int tile16 = 0, ray = 0, chunks = 0, bchunks = 0, vis = 0, cnt = 0;
+        int tile16 = 0, ray = 0, chunks = 0, vis = 0, cnt = 0;
         for (int i = 0; i < n; ++i) {
             if (groupOf[i] != g) continue;
             BatchScene S = b->host[i];
@@ -772,7 +772,6 @@ int batch_upload(vm_batch* b) {
             S.rayTileFirst = ray;
             ray += ((d.width + tw - 1) / tw) * ((d.height + th - 1) / th);
             chunks += (S.m.nBlock + 255) / 256;
-            bchunks += (S.m.nBlock + kBatchBoundsChunk - 1) / kBatchBoundsChunk;
             vis += S.m.nBlock < S.m.nEntry ? S.m.nBlock : S.m.nEntry;
             dev.push_back(S);
             ++cnt;
@@ -782,7 +781,6 @@ int batch_upload(vm_batch* b) {
         v.totalRayTiles = ray;
         v.rayTilesPerXcd = (ray + 7) / 8;
         v.maxChunks = chunks;
-        v.maxBoundsChunks = bchunks;
         v.maxVisible = vis;
         b->groups.push_back(v);
     }

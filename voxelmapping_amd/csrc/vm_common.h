@@ -80,10 +80,7 @@ struct RayBounds {
     int nTiles;
     unsigned epoch;
 };
-// allocated blocks per workgroup pass of the bounds kernels: single map
-// (few chunks, the longest is the critical path) and batch (many chunks)
-constexpr int kBoundsChunk = 64;
-constexpr int kBatchBoundsChunk = 256;
+constexpr int kBoundsChunk = 64;  // allocated blocks per workgroup pass of the single-map bounds kernel
 
 struct Intrinsics {
     float fx, fy, cx, cy;  // K(0,0), K(1,1), K(0,2), K(1,2)

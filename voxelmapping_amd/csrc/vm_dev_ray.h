@@ -499,10 +499,11 @@ struct BoundsShared {
 // march samples z in [0.1, 5] m). Most rectangles are a few tiles and their
 // thread writes them; the few larger ones are appended to an LDS list and
 // the workgroup's threads take their tiles in turn.
+// The same for blocks whose positions the caller has loaded: b is this
+// thread's block (x = kEmptyKey for none).
 template <int CH>
-__device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, int base, int A, int W, int H,
-                                        const Intrinsics& K, const Affine& Tinv, float3 c, float vs, float grow,
-                                        BoundsShared<CH>& sh) {
+__device__ inline void ray_bounds_blocks(const int4 b, const RayBounds& rb, int W, int H, const Intrinsics& K,
+                                         const Affine& Tinv, float3 c, float vs, float grow, BoundsShared<CH>& sh) {
     const int tid = threadIdx.x;
     const int tilesX = (W + 7) >> 3;
     const unsigned long long ne = (unsigned long long)(~rb.epoch) << 32, fe = (unsigned long long)rb.epoch << 32;
@@ -510,9 +511,7 @@ __device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, i
     unsigned long long* farK = rb.keys + rb.nTiles + 1;
     if (tid == 0) sh.nHeavy = 0;
     __syncthreads();
-    const int i = base + tid;
-    if (i < A) {
-        const int4 b = m.blockPos[i];
+    {
         if (b.x != kEmptyKey) {
             const float lx = (float)(b.x * 8) - 1.0f - grow, hx = (float)(b.x * 8 + 8) + grow;
             const float ly = (float)(b.y * 8) - 1.0f - grow, hy = (float)(b.y * 8 + 8) + grow;
@@ -575,6 +574,15 @@ __device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, i
         }
     }
     __syncthreads();
+}
+
+template <int CH>
+__device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, int base, int A, int W, int H,
+                                        const Intrinsics& K, const Affine& Tinv, float3 c, float vs, float grow,
+                                        BoundsShared<CH>& sh) {
+    const int i = base + (int)threadIdx.x;
+    const int4 b = i < A ? m.blockPos[i] : make_int4(kEmptyKey, 0, 0, 0);
+    ray_bounds_blocks<CH>(b, rb, W, H, K, Tinv, c, vs, grow, sh);
 }
 
 // computeNormalKernel body, image_proc.cu:74-93, for pixel (x, y) with
