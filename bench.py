@@ -456,6 +456,10 @@ def main():
                 "bytes_per_launch": int(per_launch[dom]), "avg_launch_us": round(stage_avg_us[dom], 2)}
     if traffic is not None:
         roofline["traffic_source"] = traffic_src
+    if dom == "raycast":
+        # the raycast stage is two launches: the per-tile ray-distance bounds,
+        # then the march; the events bracket both
+        roofline["stage_kernels"] = ["k_ray_bounds", "k_raycast<2>"]
     out = {
         "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1000.0 * wall / args.steps, 4), "higher_is_better": True,

@@ -135,7 +135,12 @@ int vm_check_visible_blocks(vm_map* map, int32_t width, int32_t height, const fl
                             const float cam_to_world[16], void* stream, uint32_t* out_visible_count);
 
 /* RenderScene<TVoxel> (ray_tracing.cu:403-437): raycast every pixel into
- * vmap (float4, camera frame, x = NaN on a miss). */
+ * vmap (float4, camera frame, x = NaN on a miss). A first pass over the
+ * allocated blocks bounds, per 8x8 pixel tile, the ray distances at which a
+ * sample can read an allocated voxel; rays skip the stretch before and stop
+ * (as misses) after it, with results identical to the full march. The
+ * bounds live in a per-map buffer sized at vm_map_create for images of up to
+ * 2^16 tiles (4.2 Mpixel); a larger image makes the call allocate. */
 int vm_raycast(vm_map* map, float* vmap, int64_t vmap_pitch, int32_t width, int32_t height, const float K[4],
                const float cam_to_world[16], void* stream);
 /* VoxelMapping::RenderScene (vmapping.cpp:83-104): visible-block check, then

@@ -487,8 +487,9 @@ struct BoundsShared {
     int nHeavy;
 };
 
-// Bounds of the allocated blocks [base, base + CH) of map m, by a workgroup
-// of CH threads. A block's box of march positions whose sample can read one
+// Bounds of the allocated blocks [base, base + CH) of map m, thread i < CH
+// taking block base + i, by a workgroup of at least CH threads (all of which
+// share the large rectangles). A block's box of march positions whose sample can read one
 // of its voxels is [8b - 1, 8b + 8) per axis (the -1 for the +1 corners of
 // the trilinear read), grown by `grow` voxels (ray_drift_margin). With all
 // corners in front of the camera the box projects into the hull of its
@@ -566,7 +567,7 @@ __device__ inline void ray_bounds_blocks(const int4 b, const RayBounds& rb, int 
     const int nh = sh.nHeavy;
     for (int r = 0; r < nh; ++r) {
         const TileRect t = sh.heavy[r];
-        for (int k = tid; k < t.n; k += CH) {
+        for (int k = tid; k < t.n; k += blockDim.x) {
             const int ty = k / t.w;
             const int tile = (t.ty0 + ty) * tilesX + t.tx0 + (k - ty * t.w);
             atomicMin(&nearK[tile], ne | t.nearBits);
@@ -581,7 +582,7 @@ __device__ inline void ray_bounds_chunk(const MapView& m, const RayBounds& rb, i
                                         const Intrinsics& K, const Affine& Tinv, float3 c, float vs, float grow,
                                         BoundsShared<CH>& sh) {
     const int i = base + (int)threadIdx.x;
-    const int4 b = i < A ? m.blockPos[i] : make_int4(kEmptyKey, 0, 0, 0);
+    const int4 b = (int)threadIdx.x < CH && i < A ? m.blockPos[i] : make_int4(kEmptyKey, 0, 0, 0);
     ray_bounds_blocks<CH>(b, rb, W, H, K, Tinv, c, vs, grow, sh);
 }
 

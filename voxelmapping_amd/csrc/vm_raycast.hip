@@ -37,8 +37,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
 }
 
 // Ray-distance bounds of every tile (ray_bounds_chunk), kBoundsChunk
-// allocated blocks per workgroup pass.
-__global__ __launch_bounds__(kBoundsChunk) void k_ray_bounds(MapView m, RayBounds rb, int W, int H, Intrinsics K, Affine Tinv,
+// allocated blocks per pass of a 256-thread workgroup.
+__global__ __launch_bounds__(256) void k_ray_bounds(MapView m, RayBounds rb, int W, int H, Intrinsics K, Affine Tinv,
                                                     float3 c, float vs, float grow) {
     __shared__ BoundsShared<kBoundsChunk> sh;
     const int A = allocated_count(m.ctr, m.nBlock);
@@ -111,7 +111,7 @@ hipError_t launch_ray_bounds(const MapView& m, const RayBounds& rb, int W, int H
     const float vsInv = (float)(1.0 / (double)m.voxelSize);
     const float3 c = make_float3(pose.t0 * vsInv, pose.t1 * vsInv, pose.t2 * vsInv);
     const int need = (m.nBlock + kBoundsChunk - 1) / kBoundsChunk, g = num_compute_units() * 8;
-    hipLaunchKernelGGL(k_ray_bounds, dim3(need < g ? need : g), dim3(kBoundsChunk), 0, s, m, rb, W, H, K, Tinv, c,
+    hipLaunchKernelGGL(k_ray_bounds, dim3(need < g ? need : g), dim3(256), 0, s, m, rb, W, H, K, Tinv, c,
                        m.voxelSize, grow);
     return hipGetLastError();
 }
