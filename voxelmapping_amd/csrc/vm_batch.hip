@@ -74,11 +74,11 @@ __global__ __launch_bounds__(256) void kb_allocate(BatchView B) {
     }
     const int x = (lt % S.tilesX16) * 16 + (threadIdx.x & 15);
     const int y = (lt / S.tilesX16) * 16 + (threadIdx.x >> 4);
-    if (x >= S.d.width || y >= S.d.height) return;
+    const bool in = x < S.d.width && y < S.d.height;
     DepthImage d = S.d;
     d.data += (int64_t)f * S.depthFrameStride;
-    if (d.format == kDepthU16) allocate_pixel<kDepthU16>(S.m, d, S.K, S.poses[f], x, y);
-    else allocate_pixel<kDepthF32>(S.m, d, S.K, S.poses[f], x, y);
+    if (d.format == kDepthU16) allocate_pixel<kDepthU16>(S.m, d, S.K, S.poses[f], x, y, in);
+    else allocate_pixel<kDepthF32>(S.m, d, S.K, S.poses[f], x, y, in);
 }
 
 // chunks of 256 allocated blocks of every scene, concatenated: the visible

@@ -141,40 +141,85 @@ __device__ inline float load_depth(const DepthImage& d, int x, int y) {
     else return (float)((const uint16_t*)row)[x] * d.scale;
 }
 
-// CreateBlockLineTracingFunctor::operator() for pixel (x, y), depth_fusion.cu:32-59
+// CreateBlockLineTracingFunctor::operator() for pixel (x, y),
+// depth_fusion.cu:32-59, run by a whole wave whose lanes cover a 16 x 4
+// pixel tile (lane = 16 * row + column): every lane must call this, a lane
+// without a pixel with inImage = false. Each lane walks its segment's blocks
+// in order, consecutive duplicates being no-ops as in the reference loop, and
+// keeps its first kLaneKeys distinct blocks (later ones are inserted at
+// once). Neighbouring pixels cross mostly the same blocks, so a lane skips a
+// kept block that the lane to its left or above also keeps: the lowest lane
+// keeping a block never skips it, so the inserted set is still the union of
+// the lanes' blocks, with a hash probe per block and tile row run instead of
+// per pixel.
+constexpr int kLaneKeys = 4;
+
 template <int DF>
 __device__ inline void allocate_pixel(const MapView& m, const DepthImage& d, const Intrinsics& K, const Affine& T,
-                                      int x, int y) {
-    const float dist = load_depth<DF>(d, x, y);
-    if (isnan(dist) || dist < m.depthMin || dist > m.depthMax) return;
-    const float truncHalf = m.trunc * 0.5f;  // (float)(trunc * 0.5), depth_fusion.cu:226
-    const float dNear = fmaxf(m.depthMin, dist - truncHalf);
-    const float dFar = fmaxf(m.depthMin, dist + truncHalf);
-    if (dNear >= dFar) return;
-    // unproject (map_struct.h:189-194), camToWorld, worldPtToVoxelPos (:116-120)
-    const float ux = K.ifx * ((float)x - K.cx), uy = K.ify * ((float)y - K.cy);
-    float3 wn = xform(T, make_float3(ux * dNear, uy * dNear, dNear));
-    float3 wf = xform(T, make_float3(ux * dFar, uy * dFar, dFar));
-    const float vs = m.voxelSize;
-    const int pnx = floor_i(wn.x / vs), pny = floor_i(wn.y / vs), pnz = floor_i(wn.z / vs);
-    const int pfx = floor_i(wf.x / vs), pfy = floor_i(wf.y / vs), pfz = floor_i(wf.z / vs);
-    float dx = (float)(pfx - pnx), dy = (float)(pfy - pny), dz = (float)(pfz - pnz);
-    const float len = sqrtf(sum3(dx * dx, dy * dy, dz * dz));
-    // (int)ceil(2.0 * len): 2*len is exact in float, so ceilf matches the double form
-    const int steps = f2i_sat(ceilf(2.0f * len));
-    const float den = (float)(steps - 1);
-    dx = dx / den; dy = dy / den; dz = dz / den;
-    float px = (float)pnx, py = (float)pny, pz = (float)pnz;
-    int lbx = INT_MIN, lby = INT_MIN, lbz = INT_MIN;
-    for (int s = 0; s < steps; ++s) {
-        // pt.cast<int>() truncates toward zero (depth_fusion.cu:58)
-        const int bx = vox_to_blk(f2i_sat(px)), by = vox_to_blk(f2i_sat(py)), bz = vox_to_blk(f2i_sat(pz));
-        if (bx != lbx || by != lby || bz != lbz) {  // consecutive duplicates are no-ops
-            insert_block(m, bx, by, bz);
-            lbx = bx; lby = by; lbz = bz;
+                                      int x, int y, bool inImage) {
+    const int lane = threadIdx.x & 63;
+    int kx[kLaneKeys], ky[kLaneKeys], kz[kLaneKeys];
+#pragma unroll
+    for (int k = 0; k < kLaneKeys; ++k) kx[k] = ky[k] = kz[k] = 0;
+    int n = 0;
+    if (inImage) {
+        const float dist = load_depth<DF>(d, x, y);
+        const float truncHalf = m.trunc * 0.5f;  // (float)(trunc * 0.5), depth_fusion.cu:226
+        const float dNear = fmaxf(m.depthMin, dist - truncHalf);
+        const float dFar = fmaxf(m.depthMin, dist + truncHalf);
+        if (!(isnan(dist) || dist < m.depthMin || dist > m.depthMax) && dNear < dFar) {
+            // unproject (map_struct.h:189-194), camToWorld, worldPtToVoxelPos (:116-120)
+            const float ux = K.ifx * ((float)x - K.cx), uy = K.ify * ((float)y - K.cy);
+            const float3 wn = xform(T, make_float3(ux * dNear, uy * dNear, dNear));
+            const float3 wf = xform(T, make_float3(ux * dFar, uy * dFar, dFar));
+            const float vs = m.voxelSize;
+            const int pnx = floor_i(wn.x / vs), pny = floor_i(wn.y / vs), pnz = floor_i(wn.z / vs);
+            const int pfx = floor_i(wf.x / vs), pfy = floor_i(wf.y / vs), pfz = floor_i(wf.z / vs);
+            float dx = (float)(pfx - pnx), dy = (float)(pfy - pny), dz = (float)(pfz - pnz);
+            const float len = sqrtf(sum3(dx * dx, dy * dy, dz * dz));
+            // (int)ceil(2.0 * len): 2*len is exact in float, so ceilf matches the double form
+            const int steps = f2i_sat(ceilf(2.0f * len));
+            const float den = (float)(steps - 1);
+            dx = dx / den; dy = dy / den; dz = dz / den;
+            float px = (float)pnx, py = (float)pny, pz = (float)pnz;
+            int lbx = INT_MIN, lby = INT_MIN, lbz = INT_MIN;
+            for (int s = 0; s < steps; ++s) {
+                // pt.cast<int>() truncates toward zero (depth_fusion.cu:58)
+                const int bx = vox_to_blk(f2i_sat(px)), by = vox_to_blk(f2i_sat(py)), bz = vox_to_blk(f2i_sat(pz));
+                if (bx != lbx || by != lby || bz != lbz) {  // consecutive duplicates are no-ops
+                    if (n < kLaneKeys) {
+#pragma unroll
+                        for (int k = 0; k < kLaneKeys; ++k)
+                            if (k == n) { kx[k] = bx; ky[k] = by; kz[k] = bz; }
+                        ++n;
+                    } else {
+                        insert_block(m, bx, by, bz);
+                    }
+                    lbx = bx; lby = by; lbz = bz;
+                }
+                px += dx; py += dy; pz += dz;
+            }
         }
-        px += dx; py += dy; pz += dz;
     }
+    // the kept blocks of the lanes to the left (same row) and above
+    const int nl = __shfl_up(n, 1, 64), nu = __shfl_up(n, 16, 64);
+    const bool hasL = (lane & 15) != 0, hasU = lane >= 16;
+    bool skip[kLaneKeys];
+#pragma unroll
+    for (int k = 0; k < kLaneKeys; ++k) skip[k] = false;
+#pragma unroll
+    for (int j = 0; j < kLaneKeys; ++j) {
+        const int lx = __shfl_up(kx[j], 1, 64), ly = __shfl_up(ky[j], 1, 64), lz = __shfl_up(kz[j], 1, 64);
+        const int ux = __shfl_up(kx[j], 16, 64), uy = __shfl_up(ky[j], 16, 64), uz = __shfl_up(kz[j], 16, 64);
+        const bool vl = hasL && j < nl, vu = hasU && j < nu;
+#pragma unroll
+        for (int k = 0; k < kLaneKeys; ++k)
+            skip[k] = skip[k] || (vl && kx[k] == lx && ky[k] == ly && kz[k] == lz) ||
+                      (vu && kx[k] == ux && ky[k] == uy && kz[k] == uz);
+    }
+#pragma unroll
+    for (int k = 0; k < kLaneKeys; ++k)
+        if (k < n && !skip[k]) insert_block(m, kx[k], ky[k], kz[k]);
 }
 
 // checkBlockVisible, map_struct.h:204-244

@@ -67,8 +67,7 @@ __global__ __launch_bounds__(256) void k_allocate(MapView m, DepthImage d, Intri
     }
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= d.width || y >= d.height) return;
-    allocate_pixel<DF>(m, d, K, T, x, y);
+    allocate_pixel<DF>(m, d, K, T, x, y, x < d.width && y < d.height);
 }
 
 // Visibility over the allocated blocks [0, A) instead of the whole hash
