@@ -5,7 +5,7 @@
 // Marching cubes: one 64-lane workgroup per allocated block. The block's
 // 8x8x8 voxels plus the +x/+y/+z face layers of its 7 neighbours (a 9^3
 // halo) are read once into LDS, with one hash lookup per neighbour instead of
-// 8 lookups per voxel. Each lane owns an (x, y) column of 8 cubes, counts its
+// 8 lookups per voxel; the case tables are LDS-resident too. Each lane owns an (x, y) column of 8 cubes, counts its
 // triangles, a wave prefix sum gives each lane its output range, and one
 // atomic per block reserves the block's range in the triangle buffer. Output
 // order differs from the reference (which is already nondeterministic);
@@ -55,7 +55,16 @@ __global__ __launch_bounds__(64) void k_polygonize(MapView m, float* verts, floa
     __shared__ float s_sdf[kHalo];
     __shared__ uint8_t s_ok[kHalo];
     __shared__ int s_nb[8];
+    // the case tables, read with per-lane case indices, live in LDS for the
+    // workgroup's whole life (2.5 KB, loaded once)
+    __shared__ uint64_t s_tri[256];
+    __shared__ uint16_t s_edges[256];
     const int lane = threadIdx.x;
+    for (int i = lane; i < 256; i += 64) {
+        s_tri[i] = c_mc.tri[i];
+        s_edges[i] = c_mc.edges[i];
+    }
+    __syncthreads();
     const int A = allocated_count(m.ctr, m.nBlock);
     const float vs = m.voxelSize;
     for (int b = blockIdx.x; b < A; b += gridDim.x) {
@@ -92,9 +101,9 @@ __global__ __launch_bounds__(64) void k_polygonize(MapView m, float* verts, floa
                 all = all && s_ok[h0 + cofs[c]];
                 if (s_sdf[h0 + cofs[c]] < 0.f) cube |= 1 << c;
             }
-            if (!all || c_mc.edges[cube] == 0) cube = 0;
+            if (!all || s_edges[cube] == 0) cube = 0;
             cubes[z] = cube;
-            ntri += (int)(c_mc.tri[cube] & 15u);
+            ntri += (int)(s_tri[cube] & 15u);
         }
         // wave-inclusive scan of the per-lane triangle counts
         int incl = ntri;
@@ -113,7 +122,7 @@ __global__ __launch_bounds__(64) void k_polygonize(MapView m, float* verts, floa
         if (ntri > 0) {
             for (int z = 0; z < 8; ++z) {
                 const int cube = cubes[z];
-                const uint64_t tri = c_mc.tri[cube];
+                const uint64_t tri = s_tri[cube];
                 const int nt = (int)(tri & 15u);
                 if (nt == 0) continue;
                 const int h0 = lx + 9 * ly + 81 * z;
