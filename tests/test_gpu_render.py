@@ -103,6 +103,25 @@ def test_raycast_parity_unusual_poses(fused, which):
     _compare_vmaps(gv2, ov2)
 
 
+def test_raycast_bounds_across_image_sizes(fused):
+    """One map raycast at several image sizes in turn: the per-tile bounds
+    keep their layout by capacity, and 4096x1040 (66560 tiles, more than the
+    map's initial 2^16) reallocates and refills them; every size matches the
+    oracle."""
+    import torch
+    w, g, o = fused
+    T = w.poses(10)[9]
+    for W, H in [(320, 240), (4096, 1040), (160, 120), (640, 480)]:
+        s = W / 320.0
+        K = (262.5 * s, 262.5 * s, (W - 1) / 2.0, (H - 1) / 2.0)
+        vm = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        g.raycast(vm, K, T)
+        torch.cuda.synchronize()
+        gv = vm.cpu().numpy()
+        ov = o.raycast(W, H, K, T, init=0.0)
+        assert _compare_vmaps(gv, ov) > 0.5, (W, H)
+
+
 def test_render_scene_facade_and_gate(fused):
     w, g, o = fused
     Tc = w.poses(10)[9]

@@ -130,7 +130,11 @@ int ray_bounds_for(vm_map* m, int W, int H, hipStream_t s, RayBounds* out) {
     if (n > m->rayKeysTiles || m->rayEpoch == 0xfffffffeu) {
         const int cap = std::max(n, 1 << 16);
         if (cap > m->rayKeysTiles) {
-            if (m->rayKeys) hipFree(m->rayKeys);
+            // raycasts already queued on any stream may still read the old keys
+            if (m->rayKeys) {
+                VM_HIP(hipDeviceSynchronize(), "device synchronize");
+                hipFree(m->rayKeys);
+            }
             m->rayKeys = nullptr;
             m->rayKeysTiles = 0;
             hipError_t e = hipMalloc((void**)&m->rayKeys, sizeof(unsigned long long) * 2 * ((size_t)cap + 1));
