@@ -30,6 +30,7 @@ METRIC = "frames/s fuse+raycast 640x480 @5mm, batched vs plain; % of HBM peak"
 # algorithmic bytes (per pixel / per visible block) of one frame step
 B_DEPTH, B_COLOR, B_VMAP, B_NMAP = 2, 3, 16, 16
 S_VOXEL, S_RAY, S_ENTRY = 11, 8, 16
+S_VOXEL_NO_COLOR = 8  # float TSDF + float weight when no colour is fused
 
 
 def stage_bytes(P, V):
@@ -177,7 +178,7 @@ def batched_bench(warmup: int, steps: int) -> dict:
     n = len(ws)
     P_total = sum(w.width * w.height for w in ws)
     bytes_frame_pixels = P_total * (B_DEPTH + B_VMAP + B_NMAP)  # no colour in this workload
-    per_vis = 512 * (2 * S_VOXEL + S_RAY) + S_ENTRY
+    per_vis = 512 * (2 * S_VOXEL_NO_COLOR + S_RAY) + S_ENTRY
     batch_bytes = steps * bytes_frame_pixels + v_batch * per_vis
     return {
         "workload": "16 scenes, 160x120..640x480, 4/6/10 mm, 2^16..2^20 buckets, uint16 depth, float TSDF",
@@ -262,6 +263,7 @@ def scale_bench(frames: int = 24, warmup: int = 4, track_every: int = 10, n_hyp:
     for i in range(warmup):
         step(i)
     torch.cuda.synchronize()
+    v0 = m.stats()["sum_visible"]
     t0 = time.perf_counter()
     for i in range(warmup, n):
         if i % track_every == 0:
@@ -271,8 +273,16 @@ def scale_bench(frames: int = 24, warmup: int = 4, track_every: int = 10, n_hyp:
     dt = time.perf_counter() - t0 - sum(refine_s)
     st = m.stats()
     m.release()
+    # algorithmic bytes of the fused frames (no colour): depth, vertex and
+    # normal maps per pixel, TSDF + weight read and written and raycast reads
+    # per voxel of every visible block
+    vis = st["sum_visible"] - v0
+    frame_bytes_total = frames * w.width * w.height * (B_DEPTH + B_VMAP + B_NMAP) + vis * (
+        512 * (2 * S_VOXEL_NO_COLOR + S_RAY) + S_ENTRY)
     return {"workload": w.name, "frames": [warmup, n], "value": round(frames / dt, 2), "unit": "frames/s",
-            "ms_per_step": round(1000 * dt / frames, 4), "allocated_blocks": st["allocated_blocks"],
+            "ms_per_step": round(1000 * dt / frames, 4), "mean_visible_blocks": round(vis / frames, 1),
+            "frame_hbm_frac": round(frame_bytes_total / dt / 1e9 / HBM_PEAK_GBS, 4),
+            "allocated_blocks": st["allocated_blocks"],
             "status": st["status"], "voxel_pool_gib": round(w.n_block * 512 * 12 / 2**30, 1),
             "tracking": {"every": track_every, "hypotheses": n_hyp, "iterations": iters,
                          "pixels": ((w.width + stride - 1) // stride) * ((w.height + stride - 1) // stride),
