@@ -96,11 +96,8 @@ __global__ __launch_bounds__(256) void kb_visible(BatchView B) {
         const BatchScene& S = B.scenes[s];
         if (f >= S.nFrames) continue;
         const int A = allocated_count(S.m.ctr, S.m.nBlock);
-        const DepthImage d{};
-        const ColorImage col{};
         const int base = (c - s_first[s]) * 256;
-        visible_chunk<false, kFmtRef, kDepthF32, false>(S.m, d, col, S.K, S.posesInv[f], S.d.width, S.d.height,
-                                                         base, A, sh);
+        visible_chunk(S.m, S.K, S.posesInv[f], S.d.width, S.d.height, base, A, sh);
         const Affine& T = S.poses[f];
         const float grow = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
         const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);

@@ -384,18 +384,14 @@ __device__ inline void integrate_block(const MapView& m, const DepthImage& d, co
 struct VisibleShared {
     int wave[4];
     int base;
-    int blk[256];
-    int4 key[256];
 };
 
 // One 256-block chunk of the allocated list [chunk, chunk + 256): test each
-// block (one per lane), compact the visible ones with wave ballots into LDS
-// and the map's visible table (one atomic per chunk), then (FUSE) integrate
-// them with the workgroup's four waves. Returns the chunk's visible count.
-// Must be called by all 256 threads of the workgroup.
-template <bool FUSE, int FMT, int DF, bool COLOR>
-__device__ inline int visible_chunk(const MapView& m, const DepthImage& d, const ColorImage& c, const Intrinsics& K,
-                                    const Affine& Tinv, int cols, int rows, int chunk, int A, VisibleShared& sh) {
+// block (one per lane) and compact the visible ones with wave ballots into
+// the map's visible table (one atomic per chunk). Returns the chunk's visible
+// count. Must be called by all 256 threads of the workgroup.
+__device__ inline int visible_chunk(const MapView& m, const Intrinsics& K, const Affine& Tinv, int cols, int rows,
+                                    int chunk, int A, VisibleShared& sh) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = chunk + threadIdx.x;
     bool vis = false;
@@ -414,18 +410,11 @@ __device__ inline int visible_chunk(const MapView& m, const DepthImage& d, const
     __syncthreads();
     if (vis) {
         const int r = woff + __popcll(mask & ((1ull << lane) - 1ull));
-        sh.blk[r] = b;
-        sh.key[r] = key;
         HashEntry e;
         e.ptr = b * kBlockVox;
         e.offset = m.table[key.w].offset;
         e.x = key.x; e.y = key.y; e.z = key.z;
         m.visible[sh.base + r] = e;
-    }
-    if constexpr (FUSE) {
-        __syncthreads();
-        for (int i = wave; i < total; i += 4)
-            integrate_block<FMT, DF, COLOR>(m, d, c, K, Tinv, sh.blk[i], sh.key[i], lane);
     }
     __syncthreads();
     return total;

@@ -76,10 +76,8 @@ __global__ __launch_bounds__(256) void k_allocate(MapView m, DepthImage d, Intri
 __global__ __launch_bounds__(256) void k_visible(MapView m, Intrinsics K, Affine Tinv, int cols, int rows) {
     __shared__ VisibleShared sh;
     const int A = allocated_count(m.ctr, m.nBlock);
-    const DepthImage d{};
-    const ColorImage c{};
     for (int chunk = blockIdx.x * 256; chunk < A; chunk += gridDim.x * 256)
-        visible_chunk<false, kFmtRef, kDepthF32, false>(m, d, c, K, Tinv, cols, rows, chunk, A, sh);
+        visible_chunk(m, K, Tinv, cols, rows, chunk, A, sh);
 }
 
 // Integration of the compacted visible table, one wave per visible block,
