@@ -46,16 +46,20 @@ def stage_bytes(P, V):
 def profiled_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` on `workload` measured by the two
     rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known
-    size) recorded in profiles/<kernel>_<workload>_traffic.md; None if absent."""
+    size) and the kernel's L2 hit rate (TCC_HIT_sum, TCC_MISS_sum), recorded
+    in profiles/<kernel>_<workload>_traffic.md; None where absent."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    traffic, l2 = None, None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":", 1)[1].strip()), os.path.relpath(path, ROOT)
+                    traffic = int(line.split(":", 1)[1].strip())
+                elif line.startswith("l2_hit_rate:"):
+                    l2 = float(line.split(":", 1)[1].strip())
     except OSError:
-        pass
-    return None, None
+        return None, None, None
+    return traffic, l2, os.path.relpath(path, ROOT)
 
 
 def frame_bytes(P, V):
@@ -460,12 +464,14 @@ def main():
     per_launch = stage_bytes(P, mean_vis)
     dom = max(names, key=lambda n: stage_avg_us[n])
     ach = per_launch[dom] / (stage_avg_us[dom] * 1e-6) / 1e9
-    traffic, traffic_src = profiled_traffic(dom, w.name)
+    traffic, l2_hit, traffic_src = profiled_traffic(dom, w.name)
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": int(per_launch[dom]), "avg_launch_us": round(stage_avg_us[dom], 2)}
     if traffic is not None:
         roofline["traffic_source"] = traffic_src
+    if l2_hit is not None:
+        roofline["l2_hit_rate"] = l2_hit
     if dom == "raycast":
         # the raycast stage is two launches: the per-tile ray-distance bounds,
         # then the march; the events bracket both
