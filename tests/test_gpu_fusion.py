@@ -37,6 +37,44 @@ def test_fusion_room_vga_u16(oracle_threads):
     compare_maps(g, o, 2)
 
 
+def test_room_vga_pipeline_parity(oracle_threads):
+    """The benchmark workload end to end (640x480 uint16 depth + colour, 5 mm,
+    float TSDF): 12 fused frames, then the raycast at the last pose and at an
+    earlier one, the normals and the mesh against the oracle."""
+    import torch
+    from voxelmapping_amd import computeNormal, synth
+    from parity import canonical_triangles
+    w = synth.room_vga()
+    g, o = make_pair(w)
+    fuse_both(w, g, o, range(12), rgb=True)
+    compare_maps(g, o, 2)
+    for f in (11, 5):
+        T = w.poses(12)[f]
+        vm = torch.zeros((w.height, w.width, 4), dtype=torch.float32, device="cuda")
+        g.render_scene(vm, w.K, T)
+        nm = computeNormal(vm, torch.zeros_like(vm))
+        torch.cuda.synchronize()
+        gv = vm.cpu().numpy()
+        ov = np.zeros_like(gv)
+        o.render_scene(ov, w.K, T)
+        gh, oh = ~np.isnan(gv[..., 0]), ~np.isnan(ov[..., 0])
+        assert np.array_equal(gh, oh) and gh.mean() > 0.5
+        assert np.max(np.abs(gv[gh][:, :3] - ov[oh][:, :3])) <= 1e-4
+        gn, on = nm.cpu().numpy(), O.compute_normals(ov)
+        assert np.array_equal(np.isnan(gn), np.isnan(on))
+        fin = ~np.isnan(on)
+        assert np.max(np.abs(gn[fin] - on[fin])) <= 1e-4
+    cap = 4_000_000
+    verts = torch.zeros((cap, 9), dtype=torch.float32, device="cuda")
+    norms = torch.zeros_like(verts)
+    gb, gt = g.polygonize(verts, norms)
+    ob, ot, ov_, on_ = o.polygonize(cap)
+    assert (gb, gt) == (ob, ot) and gt > 10000
+    gs, _ = canonical_triangles(verts[:gt].cpu().numpy())
+    os_, _ = canonical_triangles(ov_)
+    assert np.max(np.abs(gs - os_)) <= 1e-5
+
+
 def test_visible_table_matches_oracle(small, oracle_threads):
     g, o = make_pair(small, fmt=0)
     fuse_both(small, g, o, range(3), rgb=False)
