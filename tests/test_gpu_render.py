@@ -200,6 +200,11 @@ def test_polygonize_parity(fused):
     assert np.max(np.abs(gn[og_] - on[oo])) <= 1e-5
     pts = surface_points(verts, gt).cpu().numpy()
     assert np.max(np.abs(pts - O.surface_points(gv))) <= 1e-6
+    # the triangle order is a function of the map: a second call is identical
+    verts2 = torch.zeros_like(verts)
+    norms2 = torch.zeros_like(verts)
+    assert g.polygonize(verts2, norms2) == (gb, gt)
+    assert torch.equal(verts2[:gt], verts[:gt]) and torch.equal(norms2[:gt], norms[:gt])
 
 
 def test_polygonize_cap_clamps(fused):

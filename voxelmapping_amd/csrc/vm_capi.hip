@@ -111,6 +111,9 @@ struct vm_map {
     unsigned long long* rayKeys = nullptr;
     int rayKeysTiles = 0;
     unsigned rayEpoch = 0;
+    // work buffer of the mesh extraction (neighbour offsets and triangle
+    // counts, about 9 ints per pool block), allocated by the first polygonize
+    int* meshWork = nullptr;
 };
 
 namespace {
@@ -273,7 +276,7 @@ int vm_map_destroy(vm_map* m) {
     if (!m) return VM_OK;
     MapView& v = m->view;
     void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr, v.occ, m->scratch,
-                    m->rayKeys};
+                    m->rayKeys, m->meshWork};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete m;
@@ -492,7 +495,10 @@ int vm_polygonize(vm_map* m, float* verts, float* norms, int64_t cap, void* stre
     if (cap < 0) return fail(VM_ERR_INVALID_ARG, "cap_triangles must be >= 0");
     if (cap > 0 && (!verts || !norms)) return fail(VM_ERR_INVALID_ARG, "verts and norms must not be NULL");
     hipStream_t s = (hipStream_t)stream;
-    VM_HIP(launch_polygonize(m->view, verts, norms, cap, s), "polygonize");
+    if (!m->meshWork)
+        VM_HIP(hipMalloc((void**)&m->meshWork, polygonize_work_ints(m->view.nBlock) * sizeof(int)),
+               "mesh work allocation");
+    VM_HIP(launch_polygonize(m->view, m->meshWork, verts, norms, cap, s), "polygonize");
     int c[kCounterInts];
     VM_HIP(hipMemcpyAsync(c, m->view.ctr, sizeof(c), hipMemcpyDeviceToHost, s), "counter copy");
     VM_HIP(hipStreamSynchronize(s), "stream synchronize");

@@ -2,17 +2,24 @@
 // triangle centroids (:314-344), vertex-map normals and Phong shading
 // (image_proc.cu:8-104) for gfx950.
 //
-// Marching cubes: one 64-lane workgroup per allocated block. The block's
-// 8x8x8 voxels plus the +x/+y/+z face layers of its 7 neighbours (a 9^3
-// halo) are read once into LDS, with one hash lookup per neighbour instead of
-// 8 lookups per voxel; the case tables are LDS-resident too. Each lane owns an (x, y) column of 8 cubes, counts its
-// triangles, a wave prefix sum gives each lane its output range, and one
-// atomic per block reserves the block's range in the triangle buffer. Output
-// order differs from the reference (which is already nondeterministic);
-// triangle contents, winding and normals follow :209-242 exactly.
+// Marching cubes: four launches. k_mc_links resolves every allocated block's
+// 7 +x/+y/+z neighbours (all hash probes in flight at once); k_mc_count and
+// k_mc_emit give each 256-thread workgroup the same fixed set of blocks and
+// read each block's 8x8x8 voxels plus its neighbours' face layers (a 9^3
+// halo) into LDS, next to the LDS-resident case tables. The counting pass
+// classifies the cubes and records per-block and per-workgroup triangle
+// counts, k_mc_scan turns the latter into workgroup offsets, and the
+// emitting pass builds the triangles of the blocks that have any, spread
+// evenly over the workgroup's threads, and stores them through LDS as
+// contiguous dwords. Nothing contends for a shared counter (one atomic per
+// block on a single address cost more than the rest of the kernel), and the
+// output order is a function of the map, where the reference's depends on
+// atomic timing; triangle contents, winding and normals follow :209-242
+// exactly.
 #include "vm_common.h"
 #include "vm_dev_ray.h"
 #include "vm_launch.h"
+#include "vm_mc_edge.h"
 #include "vm_mc_table.h"
 
 namespace vmamd {
@@ -21,140 +28,277 @@ namespace {
 
 __constant__ McTables c_mc = kMcTables;
 
-// interpolateLinear, marching_cubes.cu:106-115 (the 1e-6 guards compare in double)
-__device__ inline float interp_lin(float v1, float v2) {
-    if ((double)fabsf(0.0f - v1) < 1e-6) return 0.f;
-    if ((double)fabsf(0.0f - v2) < 1e-6) return 1.f;
-    if ((double)fabsf(v1 - v2) < 1e-6) return 0.f;
-    return (0.0f - v1) / (v2 - v1);
-}
+constexpr int kHalo = 9 * 9 * 9;
+constexpr int kMcThreads = 256;           // one workgroup (4 waves) per block
+constexpr int kMcMaxTri = kBlockVox * 5;  // at most 5 triangles per cube
+constexpr int kMcGroupsPerCu = 6;         // emit pass: ~23.7 KB of LDS per workgroup
 
-// make_vertex, marching_cubes.cu:145-204: vertex of edge e of the cube at pos
-__device__ inline float3 edge_vertex(int e, float3 p, const float* s) {
-    float t;
-    switch (e) {
-        case 0: t = interp_lin(s[0], s[1]); return make_float3(p.x + t, p.y + 0.f, p.z + 0.f);
-        case 1: t = interp_lin(s[1], s[2]); return make_float3(p.x + 1.f, p.y + t, p.z + 0.f);
-        case 2: t = interp_lin(s[2], s[3]); return make_float3(p.x + (1.f - t), p.y + 1.f, p.z + 0.f);
-        case 3: t = interp_lin(s[3], s[0]); return make_float3(p.x + 0.f, p.y + (1.f - t), p.z + 0.f);
-        case 4: t = interp_lin(s[4], s[5]); return make_float3(p.x + t, p.y + 0.f, p.z + 1.f);
-        case 5: t = interp_lin(s[5], s[6]); return make_float3(p.x + 1.f, p.y + t, p.z + 1.f);
-        case 6: t = interp_lin(s[6], s[7]); return make_float3(p.x + (1.f - t), p.y + 1.f, p.z + 1.f);
-        case 7: t = interp_lin(s[7], s[4]); return make_float3(p.x + 0.f, p.y + (1.f - t), p.z + 1.f);
-        case 8: t = interp_lin(s[0], s[4]); return make_float3(p.x + 0.f, p.y + 0.f, p.z + t);
-        case 9: t = interp_lin(s[1], s[5]); return make_float3(p.x + 1.f, p.y + 0.f, p.z + t);
-        case 10: t = interp_lin(s[2], s[6]); return make_float3(p.x + 1.f, p.y + 1.f, p.z + t);
-        default: t = interp_lin(s[3], s[7]); return make_float3(p.x + 0.f, p.y + 1.f, p.z + t);
+// Voxel offsets of every allocated block and its 7 +x/+y/+z neighbours
+// (links[8 b + n], n = dx | dy << 1 | dz << 2; -1: absent; links[8 b] = -2:
+// dead block), one thread per (block, neighbour): the hash probes of the
+// mesh, all in flight at once.
+__global__ void k_mc_links(MapView m, int* links) {
+    const int A = allocated_count(m.ctr, m.nBlock);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)A * 8;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = (int)(i >> 3), n = (int)(i & 7);
+        const int4 key = m.blockPos[b];
+        int p;
+        if (key.x == kEmptyKey) p = n == 0 ? -2 : -1;
+        else if (n == 0) p = b * kBlockVox;
+        else p = find_block(m, key.x + (n & 1), key.y + ((n >> 1) & 1), key.z + (n >> 2));
+        links[i] = p;
     }
 }
 
-constexpr int kHalo = 9 * 9 * 9;
+// LDS of the counting and emitting passes
+struct McShared {
+    float sdf[kHalo];
+    uint8_t ok[kHalo];
+    uint8_t cases[kBlockVox];
+    // the case tables, read with per-thread case indices, stay in LDS for
+    // the workgroup's whole life (loaded once)
+    uint64_t tri[256];
+    // the current block's neighbour offsets and key (x, y, z, triangle
+    // count), double-buffered by iteration parity: a wave that skips a
+    // block cannot overwrite what a slower wave still reads
+    int nb[2][8];
+    int key[2][4];
+    int wsum[kMcThreads / 64];
+};
 
+__device__ inline void mc_load_tables(McShared& sh, int tid) {
+    for (int i = tid; i < 256; i += kMcThreads) sh.tri[i] = c_mc.tri[i];
+}
+
+// the 9^3 halo of the block whose neighbour offsets are sh.nb[ph], in three
+// independent rounds of loads; ends with a barrier
 template <int FMT>
-__global__ __launch_bounds__(64) void k_polygonize(MapView m, float* verts, float* norms, int64_t cap) {
-    __shared__ float s_sdf[kHalo];
-    __shared__ uint8_t s_ok[kHalo];
-    __shared__ int s_nb[8];
-    // the case tables, read with per-lane case indices, live in LDS for the
-    // workgroup's whole life (2.5 KB, loaded once)
-    __shared__ uint64_t s_tri[256];
-    __shared__ uint16_t s_edges[256];
-    const int lane = threadIdx.x;
-    for (int i = lane; i < 256; i += 64) {
-        s_tri[i] = c_mc.tri[i];
-        s_edges[i] = c_mc.edges[i];
+__device__ inline void mc_load_halo(const MapView& m, McShared& sh, int ph, int tid) {
+    float sv[3];
+    bool okv[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int i = tid + r * kMcThreads;
+        sv[r] = 1.f;
+        okv[r] = false;
+        if (i < kHalo) {
+            const int hx = i % 9, hy = (i / 9) % 9, hz = i / 81;
+            const int p = sh.nb[ph][(hx >> 3) | ((hy >> 3) << 1) | ((hz >> 3) << 2)];
+            if (p >= 0) Vox<FMT>::read(m.voxels, (int64_t)p + local_index(hx & 7, hy & 7, hz & 7), sv[r], okv[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int i = tid + r * kMcThreads;
+        if (i < kHalo) {
+            sh.sdf[i] = sv[r];
+            sh.ok[i] = okv[r];
+        }
     }
     __syncthreads();
+}
+
+// corner offsets in Bourke order (read_sdf_list, marching_cubes.cu:68-104)
+__device__ inline int mc_corner(int k) {
+    return k == 0 ? 0 : k == 1 ? 1 : k == 2 ? 1 + 9 : k == 3 ? 9 : k == 4 ? 81 : k == 5 ? 1 + 81 : k == 6 ? 1 + 9 + 81 : 9 + 81;
+}
+
+// Cases of cubes tid and tid + 256 (cube c: x = c & 7, y = (c >> 3) & 7,
+// z = c >> 6) into sh.cases, their triangle counts into nt, then a
+// block-wide scan: returns the thread's first triangle in the block and sets
+// the block's total. Ends with a barrier.
+__device__ inline int mc_classify(McShared& sh, int tid, int nt[2], int& total) {
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int c = tid + q * kMcThreads;
+        const int h0 = (c & 7) + 9 * ((c >> 3) & 7) + 81 * (c >> 6);
+        int cube = 0;
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            all = all && sh.ok[h0 + mc_corner(k)];
+            if (sh.sdf[h0 + mc_corner(k)] < 0.f) cube |= 1 << k;
+        }
+        if (!all) cube = 0;
+        sh.cases[c] = (uint8_t)cube;
+        nt[q] = (int)(sh.tri[cube] & 15u);  // 0 for cases 0 and 255 (edgeTable 0)
+    }
+    const int mine = nt[0] + nt[1];
+    int incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    if (lane == 63) sh.wsum[wave] = incl;
+    __syncthreads();
+    int off = incl - mine;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kMcThreads / 64; ++w) {
+        const int v = sh.wsum[w];
+        off += w < wave ? v : 0;
+        total += v;
+    }
+    return off;
+}
+
+// Counting pass: the triangle count of every block (blockTri[b]) and of
+// every workgroup (wgTri[w]) over the blocks b = w, w + grid, ... that the
+// emitting pass gives workgroup w.
+template <int FMT>
+__global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* links, int* blockTri,
+                                                          unsigned* wgTri) {
+    __shared__ McShared sh;
+    const int tid = threadIdx.x;
+    mc_load_tables(sh, tid);
     const int A = allocated_count(m.ctr, m.nBlock);
-    const float vs = m.voxelSize;
-    for (int b = blockIdx.x; b < A; b += gridDim.x) {
-        const int4 key = m.blockPos[b];
-        if (key.x == kEmptyKey) continue;
-        if (lane < 8) {
-            const int dx = lane & 1, dy = (lane >> 1) & 1, dz = (lane >> 2) & 1;
-            s_nb[lane] = lane == 0 ? b * kBlockVox : find_block(m, key.x + dx, key.y + dy, key.z + dz);
-        }
+    // per-lane (vector) loads: their wait is at the use, one block later
+    int nbNext = -1;
+    if ((int)blockIdx.x < A && tid < 8) nbNext = links[(int64_t)blockIdx.x * 8 + tid];
+    unsigned wgTotal = 0;
+    int ph = 0;
+    for (int b = blockIdx.x; b < A; b += gridDim.x, ph ^= 1) {
+        const int nb = nbNext;
+        const int bn = b + gridDim.x;
+        if (bn < A && tid < 8) nbNext = links[(int64_t)bn * 8 + tid];
+        if (tid < 8) sh.nb[ph][tid] = nb;
         __syncthreads();
-        for (int i = lane; i < kHalo; i += 64) {
-            const int hx = i % 9, hy = (i / 9) % 9, hz = i / 81;
-            const int nb = (hx >> 3) | ((hy >> 3) << 1) | ((hz >> 3) << 2);
-            const int p = s_nb[nb];
-            float s = 1.f;
-            bool ok = false;
-            if (p >= 0) Vox<FMT>::read(m.voxels, (int64_t)p + local_index(hx & 7, hy & 7, hz & 7), s, ok);
-            s_sdf[i] = s;
-            s_ok[i] = ok;
+        if (sh.nb[ph][0] == -2) {  // dead block
+            if (tid == 0) blockTri[b] = 0;
+            continue;
         }
-        __syncthreads();
-        const int lx = lane & 7, ly = lane >> 3;
-        // corner offsets in Bourke order (read_sdf_list, marching_cubes.cu:68-104)
-        const int cofs[8] = {0, 1, 1 + 9, 9, 81, 1 + 81, 1 + 9 + 81, 9 + 81};
-        int cubes[8];
-        int ntri = 0;
-#pragma unroll
-        for (int z = 0; z < 8; ++z) {
-            const int h0 = lx + 9 * ly + 81 * z;
-            int cube = 0;
-            bool all = true;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                all = all && s_ok[h0 + cofs[c]];
-                if (s_sdf[h0 + cofs[c]] < 0.f) cube |= 1 << c;
-            }
-            if (!all || s_edges[cube] == 0) cube = 0;
-            cubes[z] = cube;
-            ntri += (int)(s_tri[cube] & 15u);
-        }
-        // wave-inclusive scan of the per-lane triangle counts
-        int incl = ntri;
+        mc_load_halo<FMT>(m, sh, ph, tid);
+        int nt[2], total;
+        mc_classify(sh, tid, nt, total);
+        if (tid == 0) blockTri[b] = total;
+        wgTotal += (unsigned)total;
+    }
+    if (tid == 0) wgTri[blockIdx.x] = wgTotal;
+}
+
+// Exclusive scan of the n workgroup counts (one 1024-thread workgroup);
+// the sum is the triangle count.
+__global__ __launch_bounds__(1024) void k_mc_scan(const unsigned* wgTri, int n, unsigned* wgBase, int* ctr) {
+    __shared__ unsigned s_w[16];
+    __shared__ unsigned s_carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_carry = 0u;
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += 1024) {
+        const int i = c0 + tid;
+        const unsigned v = i < n ? wgTri[i] : 0u;
+        unsigned incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            const int v = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += v;
+            const unsigned x = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += x;
         }
-        const int total = __shfl(incl, 63, 64);
-        int base = 0;
-        if (total > 0) {
-            if (lane == 0) base = atomicAdd((unsigned int*)&m.ctr[kTriangleCount], (unsigned)total);
-            base = __shfl(base, 0, 64);
-        }
-        int64_t id = (int64_t)(unsigned)base + (incl - ntri);
-        if (ntri > 0) {
-            for (int z = 0; z < 8; ++z) {
-                const int cube = cubes[z];
-                const uint64_t tri = s_tri[cube];
-                const int nt = (int)(tri & 15u);
-                if (nt == 0) continue;
-                const int h0 = lx + 9 * ly + 81 * z;
-                float s[8];
-#pragma unroll
-                for (int c = 0; c < 8; ++c) s[c] = s_sdf[h0 + cofs[c]];
-                const float3 pos = make_float3((float)(key.x * 8 + lx), (float)(key.y * 8 + ly), (float)(key.z * 8 + z));
-                for (int k = 0; k < nt; ++k, ++id) {
-                    if (id >= cap) continue;
-                    const int ea = (int)((tri >> (4 + 12 * k)) & 15u);
-                    const int eb = (int)((tri >> (8 + 12 * k)) & 15u);
-                    const int ec = (int)((tri >> (12 + 12 * k)) & 15u);
-                    const float3 va = edge_vertex(ea, pos, s), vb = edge_vertex(eb, pos, s),
-                                 vc = edge_vertex(ec, pos, s);
-                    // stored reversed: (c, b, a), scaled to metres
-                    const float3 p0 = make_float3(vc.x * vs, vc.y * vs, vc.z * vs);
-                    const float3 p1 = make_float3(vb.x * vs, vb.y * vs, vb.z * vs);
-                    const float3 p2 = make_float3(va.x * vs, va.y * vs, va.z * vs);
-                    const float3 n = normalized3(cross3(make_float3(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z),
-                                                        make_float3(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z)));
-                    float* o = verts + id * 9;
-                    o[0] = p0.x; o[1] = p0.y; o[2] = p0.z;
-                    o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
-                    o[6] = p2.x; o[7] = p2.y; o[8] = p2.z;
-                    float* q = norms + id * 9;
-                    q[0] = n.x; q[1] = n.y; q[2] = n.z;
-                    q[3] = n.x; q[4] = n.y; q[5] = n.z;
-                    q[6] = n.x; q[7] = n.y; q[8] = n.z;
-                }
-            }
-        }
+        if (lane == 63) s_w[wave] = incl;
         __syncthreads();
+        unsigned off = s_carry, tot = 0u;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            const unsigned x = s_w[w];
+            off += w < wave ? x : 0u;
+            tot += x;
+        }
+        if (i < n) wgBase[i] = off + incl - v;
+        __syncthreads();
+        if (tid == 0) s_carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) ctr[kTriangleCount] = (int)s_carry;
+}
+
+// Emitting pass, one 256-thread workgroup per block over the same blocks as
+// the counting pass: blocks without triangles are skipped; for the others
+// the halo is read and classified again, an LDS map lists (cube, k) per
+// triangle, each thread builds one triangle per pass of 256, staged in LDS,
+// and the pass is stored as contiguous dwords (coalesced) at the
+// workgroup's own offset, up to `cap`. Triangle order is a function of the
+// map alone: workgroup, then block, then cube, then the case's order.
+template <int FMT>
+__global__ __launch_bounds__(kMcThreads) void k_mc_emit(MapView m, const int* links, const int* blockTri,
+                                                         const unsigned* wgBase, float* verts, float* norms,
+                                                         int64_t cap) {
+    __shared__ McShared sh;
+    __shared__ uint16_t s_map[kMcMaxTri];  // triangle -> (cube << 3) | k
+    __shared__ float s_v[kMcThreads * 9];
+    __shared__ float s_n[kMcThreads * 3];
+    const int tid = threadIdx.x;
+    mc_load_tables(sh, tid);
+    const int A = allocated_count(m.ctr, m.nBlock);
+    const float vs = m.voxelSize;
+    int64_t base = (int64_t)wgBase[blockIdx.x];
+    // per-lane (vector) loads: their wait is at the use, one block later
+    int nbNext = -1, kcNext = 0;
+    auto prefetch = [&](int bb) {
+        if (tid < 8) nbNext = links[(int64_t)bb * 8 + tid];
+        if (tid < 3) kcNext = ((const int*)m.blockPos)[(int64_t)bb * 4 + tid];
+        if (tid == 3) kcNext = blockTri[bb];
+    };
+    if ((int)blockIdx.x < A) prefetch(blockIdx.x);
+    int ph = 0;
+    for (int b = blockIdx.x; b < A; b += gridDim.x, ph ^= 1) {
+        const int nb = nbNext, kc = kcNext;
+        if (b + (int)gridDim.x < A) prefetch(b + gridDim.x);
+        if (tid < 8) sh.nb[ph][tid] = nb;
+        if (tid < 4) sh.key[ph][tid] = kc;
+        __syncthreads();
+        if (sh.key[ph][3] == 0) continue;  // no triangles (dead blocks included)
+        const int kx = sh.key[ph][0], ky = sh.key[ph][1], kz = sh.key[ph][2];
+        mc_load_halo<FMT>(m, sh, ph, tid);
+        int nt[2], total;
+        const int off = mc_classify(sh, tid, nt, total);
+        for (int k = 0; k < nt[0]; ++k) s_map[off + k] = (uint16_t)((tid << 3) | k);
+        for (int k = 0; k < nt[1]; ++k) s_map[off + nt[0] + k] = (uint16_t)(((tid + kMcThreads) << 3) | k);
+        __syncthreads();
+        for (int j0 = 0; j0 < total; j0 += kMcThreads) {
+            const int j = j0 + tid;
+            if (j < total) {
+                const int e = s_map[j], c = e >> 3, k = e & 7;
+                const uint64_t tri = sh.tri[sh.cases[c]];
+                const int lx = c & 7, ly = (c >> 3) & 7, lz = c >> 6;
+                const float* hs = sh.sdf + lx + 9 * ly + 81 * lz;
+                const float3 pos = make_float3((float)(kx * 8 + lx), (float)(ky * 8 + ly), (float)(kz * 8 + lz));
+                const int ea = (int)((tri >> (4 + 12 * k)) & 15u);
+                const int eb = (int)((tri >> (8 + 12 * k)) & 15u);
+                const int ec = (int)((tri >> (12 + 12 * k)) & 15u);
+                const float3 va = mc_edge_vertex(ea, pos, hs), vb = mc_edge_vertex(eb, pos, hs),
+                             vc = mc_edge_vertex(ec, pos, hs);
+                // stored reversed: (c, b, a), scaled to metres
+                const float3 p0 = make_float3(vc.x * vs, vc.y * vs, vc.z * vs);
+                const float3 p1 = make_float3(vb.x * vs, vb.y * vs, vb.z * vs);
+                const float3 p2 = make_float3(va.x * vs, va.y * vs, va.z * vs);
+                const float3 n = normalized3(cross3(make_float3(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z),
+                                                    make_float3(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z)));
+                float* o = s_v + tid * 9;
+                o[0] = p0.x; o[1] = p0.y; o[2] = p0.z;
+                o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
+                o[6] = p2.x; o[7] = p2.y; o[8] = p2.z;
+                s_n[tid * 3 + 0] = n.x;
+                s_n[tid * 3 + 1] = n.y;
+                s_n[tid * 3 + 2] = n.z;
+            }
+            __syncthreads();
+            // triangles [base + j0, base + j0 + n) of this pass, clipped at cap
+            const int64_t first = base + j0;
+            int64_t n = total - j0 < kMcThreads ? total - j0 : kMcThreads;
+            if (cap - first < n) n = cap - first;
+            const int words = n > 0 ? (int)n * 9 : 0;
+            float* ov = verts + first * 9;
+            float* on = norms + first * 9;
+            for (int i = tid; i < words; i += kMcThreads) {
+                const int t = i / 9, c = i - t * 9;
+                ov[i] = s_v[i];
+                on[i] = s_n[t * 3 + c % 3];
+            }
+            __syncthreads();
+        }
+        base += total;
     }
 }
 
@@ -206,17 +350,39 @@ __global__ void k_shading(const float* vmap, int64_t vpitch, const float* nmap, 
 
 }  // namespace
 
-hipError_t launch_polygonize(const MapView& m, float* verts, float* norms, int64_t cap, hipStream_t s) {
-    hipError_t e = launch_fill_u32(m.ctr + kTriangleCount, 0u, 1, s);
-    if (e != hipSuccess) return e;
-    int g = num_compute_units() * 16;
-    if (g > m.nBlock) g = m.nBlock > 0 ? m.nBlock : 1;
+static int mc_groups(int nBlock) {
+    const int g = num_compute_units() * kMcGroupsPerCu;
+    return g < nBlock ? g : (nBlock > 0 ? nBlock : 1);
+}
+
+size_t polygonize_work_ints(int nBlock) { return (size_t)nBlock * 9 + 2 * (size_t)mc_groups(nBlock); }
+
+hipError_t launch_polygonize(const MapView& m, int* work, float* verts, float* norms, int64_t cap, hipStream_t s) {
+    const int g = mc_groups(m.nBlock);
+    int* links = work;
+    int* blockTri = work + (size_t)m.nBlock * 8;
+    unsigned* wgTri = (unsigned*)(blockTri + m.nBlock);
+    unsigned* wgBase = wgTri + g;
+    hipLaunchKernelGGL(k_mc_links, dim3(num_compute_units() * 8), dim3(256), 0, s, m, links);
+    const dim3 grid(g), block(kMcThreads);
     switch (m.fmt) {
-        case kFmtRef: hipLaunchKernelGGL(k_polygonize<kFmtRef>, dim3(g), dim3(64), 0, s, m, verts, norms, cap); break;
+        case kFmtRef: hipLaunchKernelGGL(k_mc_count<kFmtRef>, grid, block, 0, s, m, links, blockTri, wgTri); break;
         case kFmtRefRGB:
-            hipLaunchKernelGGL(k_polygonize<kFmtRefRGB>, dim3(g), dim3(64), 0, s, m, verts, norms, cap);
+            hipLaunchKernelGGL(k_mc_count<kFmtRefRGB>, grid, block, 0, s, m, links, blockTri, wgTri);
             break;
-        default: hipLaunchKernelGGL(k_polygonize<kFmtF32>, dim3(g), dim3(64), 0, s, m, verts, norms, cap);
+        default: hipLaunchKernelGGL(k_mc_count<kFmtF32>, grid, block, 0, s, m, links, blockTri, wgTri);
+    }
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, wgTri, g, wgBase, m.ctr);
+    switch (m.fmt) {
+        case kFmtRef:
+            hipLaunchKernelGGL(k_mc_emit<kFmtRef>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms, cap);
+            break;
+        case kFmtRefRGB:
+            hipLaunchKernelGGL(k_mc_emit<kFmtRefRGB>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms,
+                               cap);
+            break;
+        default:
+            hipLaunchKernelGGL(k_mc_emit<kFmtF32>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms, cap);
     }
     return hipGetLastError();
 }
