@@ -7,6 +7,10 @@
  *
  * Return value: 0 on success, a negative VM_ERR_* code on failure; the
  * message of the last failure on the calling thread is vm_last_error().
+ * Calls on one map must be ordered by the caller (one stream, or events
+ * between streams): the map's internal work buffers (raycast tile bounds,
+ * mesh work, adjoint scratch) are shared by its calls, as the reference's
+ * MapStruct arrays are.
  * Unlike the reference (which prints and calls exit(0) on any CUDA error,
  * src/cuda_utils.h:16-26), errors are returned, never fatal.
  *
@@ -167,7 +171,10 @@ int vm_render_shading(const float* vmap, int64_t vmap_pitch, const float* nmap, 
 /* Polygonize<TVoxel> (marching_cubes.cu:251-305): triangle soup of every
  * allocated block into verts / norms (device, cap_triangles * 9 floats each).
  * Synchronises; returns the triangle count clamped to cap_triangles and the
- * number of allocated blocks. */
+ * number of allocated blocks. The triangle order is a function of the map
+ * (the reference's depends on atomic timing): repeated calls give identical
+ * buffers. The first call allocates a work buffer of about 36 B per pool
+ * block, kept until vm_map_destroy. */
 int vm_polygonize(vm_map* map, float* verts, float* norms, int64_t cap_triangles, void* stream,
                   uint32_t* out_block_count, uint32_t* out_triangle_count);
 /* GetSurfacePoints (marching_cubes.cu:330-344): triangle centroids. */
