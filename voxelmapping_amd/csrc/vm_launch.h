@@ -44,8 +44,8 @@ hipError_t launch_normals(const float* vmap, int64_t vpitch, float* nmap, int64_
 hipError_t launch_shading(const float* vmap, int64_t vpitch, const float* nmap, int64_t npitch,
                           uint8_t* rgba, int64_t rpitch, int width, int height, hipStream_t s);
 // BuildVertexArray / Polygonize, marching_cubes.cu:9-305; `work` holds
-// polygonize_work_ints(nBlock) ints (neighbour offsets, per-block and
-// per-workgroup triangle counts; rewritten by every call)
+// polygonize_work_ints(nBlock) ints (neighbour offsets, per-block triangle
+// counts, workgroup offsets; rewritten by every call)
 size_t polygonize_work_ints(int nBlock);
 hipError_t launch_polygonize(const MapView& m, int* work, float* verts, float* norms, int64_t capTriangles,
                              hipStream_t s);

@@ -4,11 +4,11 @@
 //
 // Marching cubes: four launches. k_mc_links resolves every allocated block's
 // 7 +x/+y/+z neighbours (all hash probes in flight at once); k_mc_count and
-// k_mc_emit give each 256-thread workgroup the same fixed set of blocks and
-// read each block's 8x8x8 voxels plus its neighbours' face layers (a 9^3
-// halo) into LDS, next to the LDS-resident case tables. The counting pass
-// classifies the cubes and records per-block and per-workgroup triangle
-// counts, k_mc_scan turns the latter into workgroup offsets, and the
+// k_mc_emit read each block's 8x8x8 voxels plus its neighbours' face layers
+// (a 9^3 halo) into 256-thread workgroups' LDS, next to the LDS-resident
+// case tables. The counting pass classifies the cubes and records per-block
+// triangle counts, k_mc_scan sums them per emitting workgroup (a fixed set
+// of blocks each) and turns the sums into workgroup offsets, and the
 // emitting pass builds the triangles of the blocks that have any, spread
 // evenly over the workgroup's threads, and stores them through LDS as
 // contiguous dwords. Nothing contends for a shared counter (one atomic per
@@ -145,12 +145,10 @@ __device__ inline int mc_classify(McShared& sh, int tid, int nt[2], int& total) 
     return off;
 }
 
-// Counting pass: the triangle count of every block (blockTri[b]) and of
-// every workgroup (wgTri[w]) over the blocks b = w, w + grid, ... that the
-// emitting pass gives workgroup w.
+// Counting pass: the triangle count of every block (blockTri[b]), over a
+// grid of its own.
 template <int FMT>
-__global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* links, int* blockTri,
-                                                          unsigned* wgTri) {
+__global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* links, int* blockTri) {
     __shared__ McShared sh;
     const int tid = threadIdx.x;
     mc_load_tables(sh, tid);
@@ -158,7 +156,6 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* l
     // per-lane (vector) loads: their wait is at the use, one block later
     int nbNext = -1;
     if ((int)blockIdx.x < A && tid < 8) nbNext = links[(int64_t)blockIdx.x * 8 + tid];
-    unsigned wgTotal = 0;
     int ph = 0;
     for (int b = blockIdx.x; b < A; b += gridDim.x, ph ^= 1) {
         const int nb = nbNext;
@@ -174,22 +171,25 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* l
         int nt[2], total;
         mc_classify(sh, tid, nt, total);
         if (tid == 0) blockTri[b] = total;
-        wgTotal += (unsigned)total;
     }
-    if (tid == 0) wgTri[blockIdx.x] = wgTotal;
 }
 
-// Exclusive scan of the n workgroup counts (one 1024-thread workgroup);
-// the sum is the triangle count.
-__global__ __launch_bounds__(1024) void k_mc_scan(const unsigned* wgTri, int n, unsigned* wgBase, int* ctr) {
+// Offsets of the emitting pass's g workgroups (one 1024-thread workgroup):
+// workgroup w takes blocks w, w + g, ..., so its count is the sum of theirs;
+// an exclusive scan over w gives the offsets, the sum the triangle count.
+__global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, unsigned* wgBase, int* ctr,
+                                                  int nBlock) {
     __shared__ unsigned s_w[16];
     __shared__ unsigned s_carry;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int A = allocated_count(ctr, nBlock);
     if (tid == 0) s_carry = 0u;
     __syncthreads();
-    for (int c0 = 0; c0 < n; c0 += 1024) {
+    for (int c0 = 0; c0 < g; c0 += 1024) {
         const int i = c0 + tid;
-        const unsigned v = i < n ? wgTri[i] : 0u;
+        unsigned v = 0u;
+        if (i < g)
+            for (int b = i; b < A; b += g) v += (unsigned)blockTri[b];
         unsigned incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned* wgTri, int n, 
             off += w < wave ? x : 0u;
             tot += x;
         }
-        if (i < n) wgBase[i] = off + incl - v;
+        if (i < g) wgBase[i] = off + incl - v;
         __syncthreads();
         if (tid == 0) s_carry += tot;
         __syncthreads();
@@ -213,8 +213,9 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const unsigned* wgTri, int n, 
     if (tid == 0) ctr[kTriangleCount] = (int)s_carry;
 }
 
-// Emitting pass, one 256-thread workgroup per block over the same blocks as
-// the counting pass: blocks without triangles are skipped; for the others
+// Emitting pass, one 256-thread workgroup per block, workgroup w taking
+// blocks w, w + grid, ... (the order k_mc_scan assumes): blocks without
+// triangles are skipped; for the others
 // the halo is read and classified again, an LDS map lists (cube, k) per
 // triangle, each thread builds one triangle per pass of 256, staged in LDS,
 // and the pass is stored as contiguous dwords (coalesced) at the
@@ -350,29 +351,28 @@ __global__ void k_shading(const float* vmap, int64_t vpitch, const float* nmap, 
 
 }  // namespace
 
-static int mc_groups(int nBlock) {
-    const int g = num_compute_units() * kMcGroupsPerCu;
+static int mc_groups(int nBlock, int perCu) {
+    const int g = num_compute_units() * perCu;
     return g < nBlock ? g : (nBlock > 0 ? nBlock : 1);
 }
 
-size_t polygonize_work_ints(int nBlock) { return (size_t)nBlock * 9 + 2 * (size_t)mc_groups(nBlock); }
+size_t polygonize_work_ints(int nBlock) { return (size_t)nBlock * 9 + (size_t)mc_groups(nBlock, kMcGroupsPerCu); }
 
 hipError_t launch_polygonize(const MapView& m, int* work, float* verts, float* norms, int64_t cap, hipStream_t s) {
-    const int g = mc_groups(m.nBlock);
+    const int g = mc_groups(m.nBlock, kMcGroupsPerCu);
+    const int gc = mc_groups(m.nBlock, 8);  // counting pass: 36 VGPRs, ~6.3 KB of LDS
     int* links = work;
     int* blockTri = work + (size_t)m.nBlock * 8;
-    unsigned* wgTri = (unsigned*)(blockTri + m.nBlock);
-    unsigned* wgBase = wgTri + g;
+    unsigned* wgBase = (unsigned*)(blockTri + m.nBlock);
     hipLaunchKernelGGL(k_mc_links, dim3(num_compute_units() * 8), dim3(256), 0, s, m, links);
-    const dim3 grid(g), block(kMcThreads);
+    const dim3 block(kMcThreads);
     switch (m.fmt) {
-        case kFmtRef: hipLaunchKernelGGL(k_mc_count<kFmtRef>, grid, block, 0, s, m, links, blockTri, wgTri); break;
-        case kFmtRefRGB:
-            hipLaunchKernelGGL(k_mc_count<kFmtRefRGB>, grid, block, 0, s, m, links, blockTri, wgTri);
-            break;
-        default: hipLaunchKernelGGL(k_mc_count<kFmtF32>, grid, block, 0, s, m, links, blockTri, wgTri);
+        case kFmtRef: hipLaunchKernelGGL(k_mc_count<kFmtRef>, dim3(gc), block, 0, s, m, links, blockTri); break;
+        case kFmtRefRGB: hipLaunchKernelGGL(k_mc_count<kFmtRefRGB>, dim3(gc), block, 0, s, m, links, blockTri); break;
+        default: hipLaunchKernelGGL(k_mc_count<kFmtF32>, dim3(gc), block, 0, s, m, links, blockTri);
     }
-    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, wgTri, g, wgBase, m.ctr);
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, blockTri, g, wgBase, m.ctr, m.nBlock);
+    const dim3 grid(g);
     switch (m.fmt) {
         case kFmtRef:
             hipLaunchKernelGGL(k_mc_emit<kFmtRef>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms, cap);
