@@ -217,6 +217,22 @@ def test_polygonize_cap_clamps(fused):
     assert g.stats()["status"] & 8
 
 
+def test_polygonize_empty_map_and_zero_cap(fused):
+    """An empty map meshes to nothing; cap_triangles = 0 accepts NULL
+    buffers and still counts the allocated blocks."""
+    import ctypes as C
+    import torch
+    from voxelmapping_amd import _capi
+    w, g, o = fused
+    e, _ = make_pair(w, fmt=2)
+    verts = torch.zeros((16, 9), dtype=torch.float32, device="cuda")
+    norms = torch.zeros_like(verts)
+    assert e.polygonize(verts, norms) == (0, 0)
+    b, t = C.c_uint32(7), C.c_uint32(7)
+    _capi.check(_capi.lib().vm_polygonize(g.handle, None, None, 0, None, C.byref(b), C.byref(t)))
+    assert (b.value, t.value) == (g.stats()["allocated_blocks"], 0)
+
+
 def test_uploaded_oracle_state_raycast_and_mesh(oracle_threads):
     """Load an oracle-built map into the GPU map and compare stage outputs."""
     import torch
