@@ -80,7 +80,7 @@ struct RayBounds {
     int nTiles;
     unsigned epoch;
 };
-constexpr int kBoundsChunk = 64;  // allocated blocks per workgroup pass of the single-map bounds kernel
+constexpr int kBoundsChunk = 32;  // allocated blocks per workgroup pass of the single-map bounds kernel
 
 struct Intrinsics {
     float fx, fy, cx, cy;  // K(0,0), K(1,1), K(0,2), K(1,2)
