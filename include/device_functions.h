@@ -81,6 +81,24 @@ void ProjectRenderingBlocks(MapStruct<TVoxel> map, unsigned int count_visible_bl
     count_rendering_block = n;
 }
 
+#ifdef VMAPPING_HAVE_OPENCV
+// The reference's signature (src/device_functions.h:35-43): the z-range maps
+// are cv::cuda::GpuMat outputs; empty ones are created at 1/8 of the image
+// size given by cols8 x rows8 = the reference facade's (width / 8) x
+// (height / 8) (vmapping.cpp:62-63).
+template <class TVoxel>
+void ProjectRenderingBlocks(MapStruct<TVoxel> map, unsigned int count_visible_block,
+                            unsigned int& count_rendering_block, cv::cuda::GpuMat& zRangeX, cv::cuda::GpuMat& zRangeY,
+                            RenderingBlock* listRenderingBlock, const Matrix4f& worldToCam, const Matrix3f& K,
+                            int cols8 = 0, int rows8 = 0) {
+    if (zRangeX.empty()) detail::ensure_output(zRangeX, rows8, cols8, CV_32FC1);
+    if (zRangeY.empty()) detail::ensure_output(zRangeY, zRangeX.rows, zRangeX.cols, CV_32FC1);
+    GpuMat zx(zRangeX), zy(zRangeY);
+    ProjectRenderingBlocks(map, count_visible_block, count_rendering_block, zx, zy, listRenderingBlock, worldToCam,
+                           K);
+}
+#endif
+
 // ray_tracing.cu:403-437 (zRange maps unused, as in the reference)
 template <class TVoxel>
 void RenderScene(MapStruct<TVoxel> map, GpuMat vmap, GpuMat /*zRangeX*/, GpuMat /*zRangeY*/,

@@ -21,6 +21,23 @@ inline void computeNormal(const GpuMat vmap, GpuMat& nmap) {
                                 vmap.cols, vmap.rows, nullptr));
 }
 
+#ifdef VMAPPING_HAVE_OPENCV
+// The reference's signatures (include/image_proc.h:10-11): an empty output is
+// created as the reference creates it (image_proc.cu:64-65, :96-97), in HIP
+// memory released with its last header.
+inline void renderScene(const cv::cuda::GpuMat vmap, const cv::cuda::GpuMat nmap, cv::cuda::GpuMat& image) {
+    detail::ensure_output(image, vmap.rows, vmap.cols, CV_8UC4);
+    GpuMat out(image);
+    renderScene(GpuMat(vmap), GpuMat(nmap), out);
+}
+
+inline void computeNormal(const cv::cuda::GpuMat vmap, cv::cuda::GpuMat& nmap) {
+    detail::ensure_output(nmap, vmap.rows, vmap.cols, vmap.type());
+    GpuMat out(nmap);
+    computeNormal(GpuMat(vmap), out);
+}
+#endif
+
 }  // namespace vmap
 
 #endif

@@ -4,7 +4,9 @@
 // and OpenCV (cv::cuda::GpuMat, cv::Mat). These types carry the same data
 // layout (column-major matrices; pitched images with data/step/rows/cols), so
 // the drop-in headers compile without Eigen or OpenCV. When those libraries
-// are available the types convert implicitly from the Eigen / OpenCV ones.
+// are available the types convert implicitly from the Eigen / OpenCV ones,
+// and the headers add overloads taking the reference's own argument types
+// (cv::cuda::GpuMat& outputs, cv::Mat host images, Eigen vectors).
 #ifndef VMAPPING_VM_TYPES_H
 #define VMAPPING_VM_TYPES_H
 
@@ -38,7 +40,12 @@ struct Vector3f {
     float& operator()(int i) { return v[i]; }
     float operator()(int i) const { return v[i]; }
     const float* data() const { return v; }
+#ifdef VMAPPING_HAVE_EIGEN
+    Vector3f(const Eigen::Vector3f& e) : v{e(0), e(1), e(2)} {}
+    operator Eigen::Vector3f() const { return Eigen::Vector3f(v[0], v[1], v[2]); }
+#endif
 };
+static_assert(sizeof(Vector3f) == 12, "Vector3f is three packed floats, like Eigen::Vector3f");
 
 struct Vector3i {
     int v[3] = {0, 0, 0};
@@ -74,8 +81,12 @@ struct MatrixNf {
 using Matrix3f = MatrixNf<3>;
 using Matrix4f = MatrixNf<4>;
 
-// OpenCV-compatible element type codes used by the drop-in API
+// OpenCV's element type codes used by the drop-in API (CV_MAKETYPE(depth,
+// channels) values). OpenCV defines them as macros with these values, so they
+// are declared here only without it.
+#ifndef VMAPPING_HAVE_OPENCV
 enum : int { CV_8UC3 = 16, CV_8UC4 = 24, CV_16UC1 = 2, CV_32FC1 = 5, CV_32FC4 = 29 };
+#endif
 
 inline size_t elem_size(int type) {
     switch (type) {
@@ -112,7 +123,9 @@ struct GpuMat {
     }
     ~GpuMat() { release(); }
 #ifdef VMAPPING_HAVE_OPENCV
+    // a non-owning view of an OpenCV device image, and back
     GpuMat(const cv::cuda::GpuMat& g) : data(g.data), step(g.step), rows(g.rows), cols(g.cols), type_(g.type()) {}
+    operator cv::cuda::GpuMat() const { return cv::cuda::GpuMat(rows, cols, type_, data, step); }
 #endif
     void create(int r, int c, int t) {
         if (data && r == rows && c == cols && t == type_) return;
@@ -129,6 +142,58 @@ struct GpuMat {
     bool empty() const { return data == nullptr; }
     int type() const { return type_; }
 };
+
+#ifdef VMAPPING_HAVE_OPENCV
+namespace detail {
+
+// cv::cuda::GpuMat::Allocator over hipMalloc / hipFree. Where the reference
+// calls GpuMat::create on an empty output (image_proc.cu:64-65, :96-97;
+// vmapping.cpp:90-91), the drop-in gives the output a buffer of this
+// allocator with OpenCV's reference count set to one: copies of the header
+// share it, and the last one to be released returns it through free().
+struct HipGpuMatAllocator final : public cv::cuda::GpuMat::Allocator {
+    bool allocate(cv::cuda::GpuMat* mat, int rows, int cols, size_t elemSize) override {
+        const size_t step = ((elemSize * (size_t)cols + 255) / 256) * 256;
+        void* p = nullptr;
+        if (hipMalloc(&p, step * (size_t)rows) != hipSuccess) return false;
+        mat->data = mat->datastart = static_cast<unsigned char*>(p);
+        mat->dataend = mat->datastart + step * (size_t)rows;
+        mat->step = step;
+        mat->refcount = new int(1);
+        return true;
+    }
+    void free(cv::cuda::GpuMat* mat) override {
+        (void)hipFree(mat->datastart);
+        delete mat->refcount;
+    }
+};
+
+inline cv::cuda::GpuMat::Allocator* hip_gpumat_allocator() {
+    static HipGpuMatAllocator a;
+    return &a;
+}
+
+// An empty `out` becomes a new rows x cols image of `type` (HIP memory);
+// a non-empty one must already have that size and type.
+inline void ensure_output(cv::cuda::GpuMat& out, int rows, int cols, int type) {
+    if (out.empty()) {
+        cv::cuda::GpuMat::Allocator* a = hip_gpumat_allocator();
+        out.allocator = a;
+        out.flags = cv::Mat::MAGIC_VAL | (type & cv::Mat::TYPE_MASK);
+        out.rows = rows;
+        out.cols = cols;
+        if (!a->allocate(&out, rows, cols, elem_size(type))) {
+            out.rows = out.cols = 0;
+            throw std::runtime_error("vmapping: hipMalloc of an output image failed");
+        }
+        return;
+    }
+    if (out.rows != rows || out.cols != cols || out.type() != type)
+        throw std::invalid_argument("vmapping: output image has the wrong size or type");
+}
+
+}  // namespace detail
+#endif
 
 }  // namespace vmap
 

@@ -49,10 +49,30 @@ def test_cpp_dropin_api_exported_and_linked():
     out = subprocess.run(["nm", "-DC", "--defined-only", _capi.LIB_PATH], capture_output=True, text=True).stdout
     for sym in ("vmap::VoxelMapping::VoxelMapping(", "vmap::VoxelMapping::CreateMap(", "vmap::VoxelMapping::FuseDepth(",
                 "vmap::VoxelMapping::FuseDepthAndImage(", "vmap::VoxelMapping::RenderScene(",
-                "vmap::VoxelMapping::Polygonize(", "vmap::VoxelMapping::GetSurfacePoints(",
-                "vmap::VoxelMapping::reset("):
+                "vmap::VoxelMapping::Polygonize(", "vmap::VoxelMapping::SurfacePoints(",
+                "vmap::VoxelMapping::FuseDepthAndImageHost(", "vmap::VoxelMapping::reset("):
         assert sym in out, sym
     demo = os.path.join(ROOT, "tests", "cpp", "dropin_demo")
     assert os.path.exists(demo), "run the build first"
     ldd = subprocess.run(["ldd", demo], capture_output=True, text=True).stdout
     assert "libvmapping_amd.so" in ldd and "not found" not in ldd.split("libvmapping_amd.so")[1].split("\n")[0]
+
+
+def test_reference_type_overloads_compile():
+    """The drop-in headers with OpenCV and Eigen present (the stand-in
+    headers of tests/cpp/stub define CV_8UC3 and its kind as macros, as OpenCV
+    does): the reference's own signatures -- cv::cuda::GpuMat& outputs,
+    FuseDepthAndImage(cv::Mat, cv::Mat, ...), std::vector<Eigen::Vector3f>
+    GetSurfacePoints() -- compile, and the program built from them links."""
+    import subprocess
+    from voxelmapping_amd import build as B
+    r = subprocess.run([B.HIPCC, "-std=c++17", "-fsyntax-only", f"-I{B.STUB_INCLUDE}", f"-I{B.INCLUDE}",
+                        B.REFTYPES_SRC], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    # without the stand-ins the same headers compile on their own types
+    r = subprocess.run([B.HIPCC, "-std=c++17", "-fsyntax-only", f"-I{B.INCLUDE}", B.DEMO_SRC],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(B.REFTYPES_BIN), "run the build first"
+    ldd = subprocess.run(["ldd", B.REFTYPES_BIN], capture_output=True, text=True).stdout
+    assert "libvmapping_amd.so" in ldd

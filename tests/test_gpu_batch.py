@@ -183,3 +183,32 @@ def test_batch_frames_exhausted_and_errors():
     with pytest.raises(VMError):
         Batch([dict(map=maps[0], K=ws[0].K, depth=torch.zeros((1, 4, 4), device="cuda"), poses=np.eye(4)[None]),
                dict(map=maps[0], K=ws[0].K, depth=torch.zeros((1, 4, 4), device="cuda"), poses=np.eye(4)[None])])
+
+
+def test_batch_rewind_after_reset():
+    """Steps, then the frame counter set back to 0 and the maps reset, then
+    steps again: the raycast's per-tile bounds of the new steps must not be
+    shadowed by keys of the earlier, later-numbered frames (the bounds' epoch
+    counts steps, not frames). Results equal the plain pipeline over the
+    rewound frames, bit for bit."""
+    import torch
+    ws = scenes_spec()
+    b, maps = build_batch(ws, 8)
+    for _ in range(6):
+        b.step()
+    torch.cuda.synchronize()
+    for m in maps:
+        m.reset()
+    b.set_frame(0)
+    for _ in range(3):
+        b.step()
+    torch.cuda.synchronize()
+    assert b.frame() == 3
+    for i, w in enumerate(ws):
+        m, vm, nm = plain_run(w, range(3))
+        same_maps(maps[i], m)
+        same_images(b.vmaps[i], vm)
+        same_images(b.nmaps[i][:-1, :-1], nm[:-1, :-1])
+        hits = (~torch.isnan(b.vmaps[i][..., 0])).float().mean().item()
+        assert hits > 0.3, f"scene {i}: hit rate {hits} after the rewind"
+    b.close()

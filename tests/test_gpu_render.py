@@ -263,3 +263,18 @@ def test_uploaded_oracle_state_raycast_and_mesh(oracle_threads):
     sg, _ = canonical_triangles(verts[:gt].cpu().numpy())
     so, _ = canonical_triangles(ov)
     assert np.max(np.abs(sg - so)) <= 1e-5
+
+
+@pytest.mark.parametrize("scale", [1.0 + 3e-5, 1.02])
+def test_raycast_non_rigid_pose(fused, scale):
+    """A camToWorld whose rotation is scaled (not orthonormal): the per-tile
+    ray-distance bounds assume a rigid rotation, so the library marches
+    these rays in full; the result still equals the oracle's march (which
+    has no bounds) -- raycast and the facade's render_scene alike."""
+    w, g, o = fused
+    T = w.poses(10)[9].reshape(4, 4).T.copy()
+    T[:3, :3] *= np.float32(scale)
+    Tc = T.T.reshape(-1)
+    ov = o.raycast(w.width, w.height, w.K, Tc, init=0.0)
+    assert _compare_vmaps(_gpu_raycast(g, w, Tc).cpu().numpy(), ov) > 0.5
+    assert _compare_vmaps(_gpu_raycast(g, w, Tc, facade=True).cpu().numpy(), ov) > 0.5

@@ -87,25 +87,41 @@ def build_synth(verbose: bool = False) -> str:
     return SYNTH
 
 
-DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_demo.cpp")
-DEMO_BIN = os.path.join(ROOT, "tests", "cpp", "dropin_demo")
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+DEMO_SRC = os.path.join(CPP_TESTS, "dropin_demo.cpp")
+DEMO_BIN = os.path.join(CPP_TESTS, "dropin_demo")
+# the same API driven with the reference's OpenCV / Eigen argument types,
+# against the stand-in headers of tests/cpp/stub
+REFTYPES_SRC = os.path.join(CPP_TESTS, "dropin_reference_types.cpp")
+REFTYPES_BIN = os.path.join(CPP_TESTS, "dropin_reference_types")
+STUB_INCLUDE = os.path.join(CPP_TESTS, "stub")
+
+
+def build_cpp_program(src: str, out: str, extra_includes=(), verbose: bool = False) -> str:
+    """C++ program using the drop-in headers, linked against the library."""
+    tmp = out + ".tmp"
+    incs = [f"-I{d}" for d in extra_includes]
+    _run([HIPCC, "-O2", "-std=c++17", *incs, f"-I{INCLUDE}", src, "-o", tmp, f"-L{PKG}", "-lvmapping_amd",
+          "-Wl,-rpath,$ORIGIN/../../voxelmapping_amd"])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
 
 
 def build_dropin_demo(verbose: bool = False) -> str:
-    """C++ program using the drop-in headers, linked against the library."""
-    tmp = DEMO_BIN + ".tmp"
-    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", DEMO_SRC, "-o", tmp, f"-L{PKG}", "-lvmapping_amd",
-          "-Wl,-rpath,$ORIGIN/../../voxelmapping_amd"])
-    os.replace(tmp, DEMO_BIN)
-    if verbose:
-        print("built", DEMO_BIN)
-    return DEMO_BIN
+    return build_cpp_program(DEMO_SRC, DEMO_BIN, verbose=verbose)
+
+
+def build_reference_types_demo(verbose: bool = False) -> str:
+    return build_cpp_program(REFTYPES_SRC, REFTYPES_BIN, [STUB_INCLUDE], verbose=verbose)
 
 
 def build_all(verbose: bool = False) -> None:
     build_synth(verbose)
     build_hip_library(verbose)
     build_dropin_demo(verbose)
+    build_reference_types_demo(verbose)
 
 
 if __name__ == "__main__":

@@ -25,7 +25,7 @@ struct BatchScene {
     int tilesX16;
     int rayTileFirst;         // first ray tile of this scene
     // per-tile ray-distance bounds (vm_dev_ray.h) of the scene's 8x8 pixel
-    // tiles; the epoch of a step is its frame index + 1
+    // tiles; the epoch of a step is the batch's step count + 1
     unsigned long long* rayKeys;
     int rayTiles;
 };
@@ -34,6 +34,9 @@ struct BatchView {
     const BatchScene* scenes;  // device array
     int nScenes;
     int* frame;                // device-side frame index
+    const unsigned* steps;     // device-side count of steps run, never rewound: the ray-bound epoch of the
+                               // next step is steps + 1, so keys of an earlier step never look current,
+                               // whatever frame the batch is set back to
     int totalTile16;
     int totalRayTiles;
     int rayTilesPerXcd;

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void kb_visible(BatchView B) {
         const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);
         const int i = base + (int)threadIdx.x;
         const int4 b = i < A ? S.m.blockPos[i] : make_int4(kEmptyKey, 0, 0, 0);
-        ray_bounds_blocks<256>(b, RayBounds{S.rayKeys, S.rayTiles, (unsigned)f + 1u}, S.d.width, S.d.height, S.K,
+        ray_bounds_blocks<256>(b, RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u}, S.d.width, S.d.height, S.K,
                                S.posesInv[f], cc, S.m.voxelSize, grow, bsh);
     }
 }
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void kb
     const Affine& T = S.poses[f];
     const float margin = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
     float nearLim, farLim;
-    ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, (unsigned)f + 1u}, (y >> 3) * ((S.d.width + 7) >> 3) + (x >> 3),
+    ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u}, (y >> 3) * ((S.d.width + 7) >> 3) + (x >> 3),
                       margin, nearLim, farLim);
     float* out = (float*)((uint8_t*)S.vmap + (int64_t)y * S.vpitch) + x * 4;
     raycast_pixel<FMT>(S.m, out, x, y, S.K, T, S.posesInv[f], S.vsInv, S.rtStep, nearLim, farLim);
@@ -180,8 +180,12 @@ __global__ __launch_bounds__(256) void kb_normals(BatchView B) {
     normal_pixel(S.vmap, S.vpitch, S.nmap, S.npitch, W, H, x, y);
 }
 
+// frame[0]: the frame index; frame[1]: the step count (the ray-bound epoch)
 __global__ void kb_advance(int* frame) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *frame += 1;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        frame[0] += 1;
+        frame[1] += 1;
+    }
 }
 
 __global__ void kb_set_frame(int* frame, int v) {

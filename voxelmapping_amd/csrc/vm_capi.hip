@@ -157,6 +157,19 @@ int ray_bounds_for(vm_map* m, int W, int H, hipStream_t s, RayBounds* out) {
     return VM_OK;
 }
 
+// The rotation block of a column-major camToWorld is orthonormal: every
+// entry of R^T R - I within 1e-5 (float roundings of a rotation sit near
+// 1e-7). The raycast's per-tile ray-distance bounds assume it.
+bool rigid_rotation(const float* T) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double d = 0.0;
+            for (int k = 0; k < 3; ++k) d += (double)T[i * 4 + k] * (double)T[j * 4 + k];
+            if (std::fabs(d - (i == j ? 1.0 : 0.0)) > 1e-5) return false;
+        }
+    return true;
+}
+
 int check_pose_k(const float* K, const float* T) {
     if (!K || !T) return fail(VM_ERR_INVALID_ARG, "K and pose must not be NULL");
     if (!finite_all(K, 4) || !finite_all(T, 16)) return fail(VM_ERR_INVALID_ARG, "K and pose must be finite");
@@ -428,7 +441,7 @@ int vm_raycast(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H, cons
     RayBounds rb;
     if (int rc = ray_bounds_for(m, W, H, (hipStream_t)stream, &rb)) return rc;
     VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, intrinsics_of(K), affine_of(T), affine_of(Ti), false,
-                          (hipStream_t)stream, &rb),
+                          (hipStream_t)stream, rigid_rotation(T) ? &rb : nullptr),
            "raycast");
     return VM_OK;
 }
@@ -449,7 +462,9 @@ int vm_render_scene(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H,
     }
     RayBounds rb;
     if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
-    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s, &rb), "raycast");
+    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s,
+                          rigid_rotation(T) ? &rb : nullptr),
+           "raycast");
     return VM_OK;
 }
 
@@ -502,13 +517,14 @@ int vm_polygonize(vm_map* m, float* verts, float* norms, int64_t cap, void* stre
     int c[kCounterInts];
     VM_HIP(hipMemcpyAsync(c, m->view.ctr, sizeof(c), hipMemcpyDeviceToHost, s), "counter copy");
     VM_HIP(hipStreamSynchronize(s), "stream synchronize");
-    const int64_t tris = (int64_t)(uint32_t)c[kTriangleCount];
-    if (tris > cap) {
+    const int64_t tris = (int64_t)((uint64_t)(uint32_t)c[kTriangleTotalLo] | ((uint64_t)(uint32_t)c[kTriangleTotalHi] << 32));
+    const int64_t kept = std::min<int64_t>(std::min<int64_t>(tris, cap), 0xffffffffll);
+    if (tris > kept) {
         int st = c[kStatus] | kStatusMeshClamped;
         VM_HIP(hipMemcpy(m->view.ctr + kStatus, &st, sizeof(int), hipMemcpyHostToDevice), "status update");
     }
     if (out_blocks) *out_blocks = (uint32_t)(m->view.nBlock - 1 - c[kHeapPtr]);
-    if (out_tris) *out_tris = (uint32_t)(tris < cap ? tris : cap);
+    if (out_tris) *out_tris = (uint32_t)kept;
     return VM_OK;
 }
 
@@ -543,6 +559,7 @@ int vm_raycast_backward(vm_map* m, const float* vmap, int64_t vpitch, const floa
     VM_HIP(launch_fill_u32(stats, 0u, 2, s), "stats reset");
     RayBounds rb;
     if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
+    if (!rigid_rotation(T)) rb.keys = nullptr;  // no bounds: the forward raycast marches in full too
     VM_HIP(launch_raycast_backward(m->view, vmap, vpitch, gv, gvpitch, gn, gnpitch, W, H, intrinsics_of(K),
                                    affine_of(T), affine_of(Ti), grad_tsdf, grad_pose, m->scratch, stats, rb, s),
            "raycast backward");
@@ -724,7 +741,10 @@ struct vm_batch {
     BatchScene* dScenes = nullptr;
     Affine* dPoses = nullptr;  // per scene: nFrames camToWorld, then nFrames worldToCam
     unsigned long long* dRayKeys = nullptr;  // per scene: 2 (tiles + 1) ray-bound keys
-    int* dFrame = nullptr;
+    int* dFrame = nullptr;  // [0] frame index, [1] steps run (ray-bound epoch - 1)
+    std::vector<size_t> keyOff;     // first key of each scene in dRayKeys
+    std::vector<size_t> keyHalf;    // near (= far) keys of each scene: tiles + 1
+    uint64_t hostSteps = 0;         // steps issued since the keys were last filled
     std::vector<BatchView> groups;  // one view per scene group (contiguous in dScenes)
     BatchTuning tuning{8, 1, 4, 8, 1};
     int fmt = 0, df = 0;
@@ -771,6 +791,7 @@ int batch_upload(vm_batch* b) {
         BatchView v{};
         v.scenes = b->dScenes + dev.size();
         v.frame = b->dFrame;
+        v.steps = (const unsigned*)(b->dFrame + 1);
         int tile16 = 0, ray = 0, chunks = 0, vis = 0, cnt = 0;
         for (int i = 0; i < n; ++i) {
             if (groupOf[i] != g) continue;
@@ -827,6 +848,21 @@ hipError_t batch_launch_all(vm_batch* b, hipStream_t s) {
         if ((e = hipStreamWaitEvent(s, b->join[g - 1], 0)) != hipSuccess) return e;
     }
     return batch_advance(b->dFrame, s);
+}
+
+// Fill every scene's ray-bound keys with "no block" for every epoch (near
+// halves all ones, far halves zero) and zero the device step count, on s.
+hipError_t batch_reset_keys(vm_batch* b, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < b->keyOff.size() && e == hipSuccess; ++i) {
+        unsigned long long* k = b->dRayKeys + b->keyOff[i];
+        const int64_t words = 2 * (int64_t)b->keyHalf[i];  // 32-bit words per half
+        e = launch_fill_u32(k, 0xffffffffu, words, s);
+        if (e == hipSuccess) e = launch_fill_u32(k + b->keyHalf[i], 0u, words, s);
+    }
+    if (e == hipSuccess) e = batch_set_frame(b->dFrame + 1, 0, s);
+    b->hostSteps = 0;
+    return e;
 }
 
 void batch_touch_maps(vm_batch* b) {
@@ -892,32 +928,32 @@ int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
                 return fail(VM_ERR_INVALID_ARG, "scene " + std::to_string(i) + ": pose " + std::to_string(f) +
                                                     " is not finite or singular");
             }
+            if (!rigid_rotation(T)) {
+                delete b;
+                return fail(VM_ERR_INVALID_ARG, "scene " + std::to_string(i) + ": pose " + std::to_string(f) +
+                                                    " has a rotation that is not orthonormal (|R^T R - I| > 1e-5)");
+            }
             poses[off + f] = affine_of(T);
             poses[off + d.n_frames + f] = affine_of(Ti);
         }
         off += 2 * (size_t)d.n_frames;
     }
-    // ray-bound keys: near halves all ones and far halves zero read as "no
-    // block" for every epoch (an epoch is a frame index + 1)
+    // ray-bound keys of every scene (2 (tiles + 1) each), filled with "no
+    // block" for every epoch; an epoch is the step count + 1, from 1 on
     size_t nKeys = 0;
-    std::vector<size_t> keyOff(n);
     for (int i = 0; i < n; ++i) {
-        keyOff[i] = nKeys;
-        nKeys += 2 * ((size_t)ray_bounds_tiles(scenes[i].width, scenes[i].height) + 1);
-    }
-    std::vector<unsigned long long> keys(nKeys, 0ull);
-    for (int i = 0; i < n; ++i) {
-        const size_t half = (size_t)ray_bounds_tiles(scenes[i].width, scenes[i].height) + 1;
-        std::fill(keys.begin() + keyOff[i], keys.begin() + keyOff[i] + half, ~0ull);
+        b->keyOff.push_back(nKeys);
+        b->keyHalf.push_back((size_t)ray_bounds_tiles(scenes[i].width, scenes[i].height) + 1);
+        nKeys += 2 * b->keyHalf.back();
     }
     hipError_t e = hipMalloc((void**)&b->dPoses, sizeof(Affine) * nPoses);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dRayKeys, sizeof(unsigned long long) * nKeys);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->dRayKeys, keys.data(), sizeof(unsigned long long) * nKeys, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dScenes, sizeof(BatchScene) * n);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dFrame, sizeof(int) * 64);
     if (e == hipSuccess) e = hipMemcpy(b->dPoses, poses.data(), sizeof(Affine) * nPoses, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(b->dFrame, 0, sizeof(int) * 64);
+    if (e == hipSuccess) e = batch_reset_keys(b, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         vm_batch_destroy(b);
         return hip_fail(e, "batch allocation");
@@ -944,7 +980,7 @@ int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
         S.npitch = d.nmap_pitch;
         S.vsInv = (float)(1.0 / (double)S.m.voxelSize);
         S.rtStep = S.m.trunc / S.m.voxelSize;
-        S.rayKeys = b->dRayKeys + keyOff[i];
+        S.rayKeys = b->dRayKeys + b->keyOff[i];
         S.rayTiles = ray_bounds_tiles(d.width, d.height);
         b->host.push_back(S);
         off += 2 * (size_t)d.n_frames;
@@ -992,6 +1028,9 @@ int vm_batch_get_frame(vm_batch* b, void* stream, int32_t* frame) {
 int vm_batch_step(vm_batch* b, void* stream) {
     if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
     hipStream_t s = (hipStream_t)stream;
+    // the 32-bit epoch (step count + 1) must never wrap onto the fill
+    // pattern: refill the keys long before it could
+    if (++b->hostSteps >= 0xfffff000ull) VM_HIP(batch_reset_keys(b, s), "ray-bound key reset");
     if (b->exec) VM_HIP(hipGraphLaunch(b->exec, s), "graph launch");
     else VM_HIP(batch_launch_all(b, s), "batch step");
     batch_touch_maps(b);

@@ -34,11 +34,13 @@ enum Counter : int {
     kExcessPtr = 1,    // next excess-list offset (map_struct.h:60)
     kVisibleCount = 2, // compacted visible entries (map_struct.h:67)
     kStatus = 3,       // sticky status bits, see StatusBits
-    kTriangleCount = 4,
+    kTriangleCount = 4,  // last mesh's triangle count, saturated at INT_MAX
     kRenderBlockCount = 5,
     kFramesFused = 6,
     kSumVisibleLo = 8, // 64-bit running sum of visible counts (stats)
     kSumVisibleHi = 9,
+    kTriangleTotalLo = 10,  // 64-bit triangle count of the last mesh
+    kTriangleTotalHi = 11,
     kCounterInts = 16,
 };
 

@@ -433,6 +433,12 @@ __device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y,
 // keys in the same order (2 (n + 1) words), n = RayBounds::nTiles (the
 // capacity, at least the image's tile count); struct RayBounds, vm_common.h.
 
+// The bounds compare a box's world distance from the camera centre with the
+// march's dist_s, measured along the camera-frame ray: they hold for a pose
+// whose rotation is orthonormal. Callers pass no keys for other poses
+// (|R^T R - I| above 1e-5, which moves a distance of at most 2500 voxels by
+// less than 0.05 voxel, inside the one voxel of slack below).
+//
 // Bound, in voxels, on how far the march's rounded positions and its dist_s
 // drift from the exact ray: at most dEnd additions (every step is >= 1
 // voxel) round a coordinate of magnitude <= cmax by <= ulp(cmax) / 2 each and
@@ -457,8 +463,14 @@ __host__ __device__ inline float ray_drift_margin(int W, int H, const Intrinsics
     return 1.0f + 1.75f * dEnd * (0.5f * ulp + 0x1p-23f) + 8.0f * ulp;
 }
 
-// [nearLim, farLim] of tile t for raycast_pixel
+// [nearLim, farLim] of tile t for raycast_pixel; without keys (a pose whose
+// rotation is not orthonormal) the whole ray
 __device__ inline void ray_bounds_limits(const RayBounds& rb, int t, float margin, float& nearLim, float& farLim) {
+    if (!rb.keys) {
+        nearLim = 0.0f;
+        farLim = INFINITY;
+        return;
+    }
     const unsigned long long* k = rb.keys;
     const int n1 = rb.nTiles + 1;
     const unsigned long long nt = k[t], ng = k[rb.nTiles], ft = k[n1 + t], fg = k[n1 + rb.nTiles];

@@ -6,6 +6,7 @@
 // std::runtime_error instead of exiting the process.
 #include <hip/hip_runtime_api.h>
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -32,6 +33,9 @@ struct VoxelMapping::VoxelizationImpl {
     float* norms_gpu = nullptr;
     float* points_gpu = nullptr;
     std::vector<float> verts_cpu, norms_cpu;
+    int fmt = -1;
+    bool warnedNoColour = false;
+    GpuMat rgbUp, depthUp;  // device copies of host images (FuseDepthAndImageHost)
 
     VoxelizationImpl(int w, int h, const Matrix3f& k) : width(w), height(h) {
         K[0] = k(0, 0);
@@ -60,6 +64,7 @@ struct VoxelMapping::VoxelizationImpl {
         d.voxel_format = fmt;
         d.max_weight = 255.f;
         vm_check(vm_map_create(&d, &map));
+        this->fmt = fmt;
     }
 
     vm_map* need_map() {
@@ -101,10 +106,42 @@ void VoxelMapping::FuseDepth(GpuMat depth, const Matrix4f& camToWorld) {
 
 void VoxelMapping::FuseDepthAndImage(GpuMat rgb, GpuMat depth, const Matrix4f& camToWorld) {
     const int fmt = depth.type() == CV_16UC1 ? VM_DEPTH_U16 : VM_DEPTH_F32;
+    if (depth.type() != CV_16UC1 && depth.type() != CV_32FC1)
+        throw std::invalid_argument("vmapping: depth must be CV_32FC1 or CV_16UC1");
     if (rgb.type() != CV_8UC3) throw std::invalid_argument("vmapping: rgb must be CV_8UC3");
-    vm_check(vm_fuse_depth(impl->need_map(), depth.data, (int64_t)depth.step, depth.cols, depth.rows, fmt, 0.001f,
-                           rgb.data, (int64_t)rgb.step, impl->K, camToWorld.data(), nullptr, nullptr));
+    if (rgb.rows != depth.rows || rgb.cols != depth.cols)
+        throw std::invalid_argument("vmapping: rgb and depth sizes differ");
+    vm_map* m = impl->need_map();
+    if (impl->fmt == VM_VOXEL_I16_U8 && !impl->warnedNoColour) {
+        std::fprintf(stderr, "vmapping: FuseDepthAndImage on a map without colour (CreateMap stores vmap::Voxel); "
+                             "the depth is fused, the colour is not (CreateMapRGB stores colour)\n");
+        impl->warnedNoColour = true;
+    }
+    vm_check(vm_fuse_depth(m, depth.data, (int64_t)depth.step, depth.cols, depth.rows, fmt, 0.001f, rgb.data,
+                           (int64_t)rgb.step, impl->K, camToWorld.data(), nullptr, nullptr));
 }
+
+void VoxelMapping::FuseDepthAndImageHost(const void* rgb, size_t rgbStep, int rgbType, const void* depth,
+                                         size_t depthStep, int depthType, int rows, int cols,
+                                         const Matrix4f& camToWorld) {
+    if (!rgb || !depth || rows <= 0 || cols <= 0) throw std::invalid_argument("vmapping: empty host image");
+    if (rgbType != CV_8UC3) throw std::invalid_argument("vmapping: rgb must be CV_8UC3");
+    if (depthType != CV_16UC1 && depthType != CV_32FC1)
+        throw std::invalid_argument("vmapping: depth must be CV_32FC1 or CV_16UC1");
+    impl->rgbUp.create(rows, cols, CV_8UC3);
+    impl->depthUp.create(rows, cols, depthType);
+    const size_t dRow = elem_size(depthType) * (size_t)cols;
+    hip_check(hipMemcpy2D(impl->rgbUp.data, impl->rgbUp.step, rgb, rgbStep, 3 * (size_t)cols, rows,
+                          hipMemcpyHostToDevice),
+              "rgb upload");
+    hip_check(hipMemcpy2D(impl->depthUp.data, impl->depthUp.step, depth, depthStep, dRow, rows,
+                          hipMemcpyHostToDevice),
+              "depth upload");
+    FuseDepthAndImage(impl->rgbUp, impl->depthUp, camToWorld);
+}
+
+int VoxelMapping::width() const { return impl->width; }
+int VoxelMapping::height() const { return impl->height; }
 
 // vmapping.cpp:83-104: visibility, then the raycast unless nothing is
 // visible; the gate is evaluated on the device (no host round trip).
@@ -130,7 +167,7 @@ int VoxelMapping::Polygonize(float*& verts_out, float*& norms_out) {
 }
 
 // vmapping.cpp:134-147
-std::vector<Vector3f> VoxelMapping::GetSurfacePoints() {
+std::vector<Vector3f> VoxelMapping::SurfacePoints() {
     const uint32_t n = impl->create_mesh();
     if (n == 0) return {};
     if (!impl->points_gpu)

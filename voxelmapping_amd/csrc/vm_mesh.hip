@@ -177,32 +177,35 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* l
 // Offsets of the emitting pass's g workgroups (one 1024-thread workgroup):
 // workgroup w takes blocks w, w + g, ..., so its count is the sum of theirs;
 // an exclusive scan over w gives the offsets, the sum the triangle count.
-__global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, unsigned* wgBase, int* ctr,
+// Offsets and the total are 64-bit (2^22 blocks of up to 2560 triangles
+// exceed 2^32); a workgroup's own sum, at most 2560 per block of its share,
+// fits 32 bits.
+__global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, unsigned long long* wgBase, int* ctr,
                                                   int nBlock) {
-    __shared__ unsigned s_w[16];
-    __shared__ unsigned s_carry;
+    __shared__ unsigned long long s_w[16];
+    __shared__ unsigned long long s_carry;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int A = allocated_count(ctr, nBlock);
-    if (tid == 0) s_carry = 0u;
+    if (tid == 0) s_carry = 0ull;
     __syncthreads();
     for (int c0 = 0; c0 < g; c0 += 1024) {
         const int i = c0 + tid;
         unsigned v = 0u;
         if (i < g)
             for (int b = i; b < A; b += g) v += (unsigned)blockTri[b];
-        unsigned incl = v;
+        unsigned long long incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            const unsigned x = __shfl_up(incl, d, 64);
+            const unsigned long long x = __shfl_up(incl, d, 64);
             if (lane >= d) incl += x;
         }
         if (lane == 63) s_w[wave] = incl;
         __syncthreads();
-        unsigned off = s_carry, tot = 0u;
+        unsigned long long off = s_carry, tot = 0ull;
 #pragma unroll
         for (int w = 0; w < 16; ++w) {
-            const unsigned x = s_w[w];
-            off += w < wave ? x : 0u;
+            const unsigned long long x = s_w[w];
+            off += w < wave ? x : 0ull;
             tot += x;
         }
         if (i < g) wgBase[i] = off + incl - v;
@@ -210,7 +213,12 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, un
         if (tid == 0) s_carry += tot;
         __syncthreads();
     }
-    if (tid == 0) ctr[kTriangleCount] = (int)s_carry;
+    if (tid == 0) {
+        const unsigned long long t = s_carry;
+        ctr[kTriangleCount] = t > 0x7fffffffull ? 0x7fffffff : (int)t;
+        ctr[kTriangleTotalLo] = (int)(unsigned)t;
+        ctr[kTriangleTotalHi] = (int)(unsigned)(t >> 32);
+    }
 }
 
 // Emitting pass, one 256-thread workgroup per block, workgroup w taking
@@ -223,7 +231,7 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, un
 // map alone: workgroup, then block, then cube, then the case's order.
 template <int FMT>
 __global__ __launch_bounds__(kMcThreads) void k_mc_emit(MapView m, const int* links, const int* blockTri,
-                                                         const unsigned* wgBase, float* verts, float* norms,
+                                                         const unsigned long long* wgBase, float* verts, float* norms,
                                                          int64_t cap) {
     __shared__ McShared sh;
     __shared__ uint16_t s_map[kMcMaxTri];  // triangle -> (cube << 3) | k
@@ -356,14 +364,18 @@ static int mc_groups(int nBlock, int perCu) {
     return g < nBlock ? g : (nBlock > 0 ? nBlock : 1);
 }
 
-size_t polygonize_work_ints(int nBlock) { return (size_t)nBlock * 9 + (size_t)mc_groups(nBlock, kMcGroupsPerCu); }
+// links (8 per block), blockTri (1 per block), then the emitting
+// workgroups' 64-bit offsets at an even int index
+size_t polygonize_work_ints(int nBlock) {
+    return (size_t)nBlock * 9 + 1 + 2 * (size_t)mc_groups(nBlock, kMcGroupsPerCu);
+}
 
 hipError_t launch_polygonize(const MapView& m, int* work, float* verts, float* norms, int64_t cap, hipStream_t s) {
     const int g = mc_groups(m.nBlock, kMcGroupsPerCu);
     const int gc = mc_groups(m.nBlock, 8);  // counting pass: 36 VGPRs, ~6.3 KB of LDS
     int* links = work;
     int* blockTri = work + (size_t)m.nBlock * 8;
-    unsigned* wgBase = (unsigned*)(blockTri + m.nBlock);
+    unsigned long long* wgBase = (unsigned long long*)(work + (((size_t)m.nBlock * 9 + 1) & ~(size_t)1));
     hipLaunchKernelGGL(k_mc_links, dim3(num_compute_units() * 8), dim3(256), 0, s, m, links);
     const dim3 block(kMcThreads);
     switch (m.fmt) {

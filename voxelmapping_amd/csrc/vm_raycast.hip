@@ -108,6 +108,7 @@ __global__ void k_zrange_init(float* zx, float* zy, int64_t pitch, int cols8, in
 
 hipError_t launch_ray_bounds(const MapView& m, const RayBounds& rb, int W, int H, const Intrinsics& K,
                              const Affine& pose, const Affine& Tinv, float grow, hipStream_t s) {
+    if (!rb.keys) return hipSuccess;  // no bounds: every ray marches in full
     const float vsInv = (float)(1.0 / (double)m.voxelSize);
     const float3 c = make_float3(pose.t0 * vsInv, pose.t1 * vsInv, pose.t2 * vsInv);
     const int need = (m.nBlock + kBoundsChunk - 1) / kBoundsChunk, g = num_compute_units() * 8;
