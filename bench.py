@@ -93,34 +93,14 @@ def cpu_baseline(w, seconds: float) -> dict:
                       f"({el:.1f} s, OpenMP integration/raycast, serial allocation)"}
 
 
-def batched_scenes(n=16, seed=4):
-    """16 seeded scenes: resolution from {160x120, 320x240, 640x480}, voxel
-    size from {4, 6, 10} mm (truncation 4 voxels), 2^16..2^20 buckets by image
-    size and voxel size, uint16 mm depth, float TSDF, random-walk trajectory."""
-    from voxelmapping_amd import synth
-    rng = np.random.default_rng(seed)
-    res = [(160, 120), (320, 240), (640, 480)]
-    out = []
-    for i in range(n):
-        W, H = res[rng.integers(3)]
-        vs = [0.004, 0.006, 0.010][rng.integers(3)]
-        f = 525.0 * W / 640
-        lb = {160: 16, 320: 18, 640: 20}[W] - (1 if vs >= 0.010 else 0) + (1 if vs <= 0.004 else 0)
-        lb = max(16, min(20, lb))
-        out.append(synth.room_vga(seed=100 + i, width=W, height=H, K=(f, f, W / 2 - 0.5, H / 2 - 0.5),
-                                  voxel_size=vs, trunc=4 * vs, n_bucket=1 << lb, n_excess=1 << (lb - 3),
-                                  n_block=1 << min(lb, 18), voxel_format=2, name=f"scene{i}_{W}x{H}_{vs * 1000:.0f}mm"))
-    return out
-
-
 def batched_bench(warmup: int, steps: int) -> dict:
     """Plain per-scene loop vs one batched step (autotuned, hipGraph) over the
     same frames of the 16-scene workload; aggregate frames/s of both."""
     import torch
-    from voxelmapping_amd import _capi
+    from voxelmapping_amd import _capi, synth
     from voxelmapping_amd.mapping import Batch, DeviceMap, intrinsics, pose_colmajor
     L = _capi.lib()
-    ws = batched_scenes()
+    ws = synth.batched16()
     nfr = warmup + steps
     dev = torch.device("cuda:0")
     maps, specs, plain_args = [], [], []

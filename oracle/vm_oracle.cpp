@@ -783,71 +783,91 @@ void or_render_shading(const float* vmap, const float* nmap, uint8_t* rgba, int 
     }
 }
 
-/* Polygonize / BuildVertexArray, marching_cubes.cu:9-305 */
+/* Polygonize / BuildVertexArray, marching_cubes.cu:9-305, for one block:
+ * emits its triangles at verts/norms + 9 * first (skipping indices >= cap)
+ * and returns how many it has. With verts == NULL it only counts. */
+static int64_t mc_block(or_map* m, const Entry& e, float* verts, float* norms, int64_t first, int64_t cap) {
+    const float vs = m->voxelSize;
+    I3 base{e.x * kBlock, e.y * kBlock, e.z * kBlock};
+    int64_t count = 0;
+    for (int lz = 0; lz < kBlock; ++lz)
+    for (int ly = 0; ly < kBlock; ++ly)
+    for (int lx = 0; lx < kBlock; ++lx) {
+        V3 pos{(float)(base.x + lx), (float)(base.y + ly), (float)(base.z + lz)};
+        /* read_sdf_list :68-104, corners in Bourke order */
+        static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                      {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+        float s[8];
+        bool ok = true;
+        for (int c = 0; c < 8 && ok; ++c) {
+            bool v;
+            s[c] = read_sdf(m, V3{pos.x + (float)off[c][0], pos.y + (float)off[c][1],
+                                  pos.z + (float)off[c][2]}, &v);
+            ok = v;
+        }
+        if (!ok) continue;
+        int cube = 0;
+        for (int c = 0; c < 8; ++c) if (s[c] < 0) cube |= 1 << c;
+        int et = g_edge_table[cube];
+        if (et == 0) continue;
+        if (!verts) {
+            for (int i = 0; g_tri_table[cube][i] != -1; i += 3) ++count;
+            continue;
+        }
+        V3 vt[12];
+        /* make_vertex :117-207 */
+        if (et & 1)    { float t = interp_lin(s[0], s[1]); vt[0]  = V3{pos.x + t, pos.y + 0.f, pos.z + 0.f}; }
+        if (et & 2)    { float t = interp_lin(s[1], s[2]); vt[1]  = V3{pos.x + 1.f, pos.y + t, pos.z + 0.f}; }
+        if (et & 4)    { float t = interp_lin(s[2], s[3]); vt[2]  = V3{pos.x + (1.f - t), pos.y + 1.f, pos.z + 0.f}; }
+        if (et & 8)    { float t = interp_lin(s[3], s[0]); vt[3]  = V3{pos.x + 0.f, pos.y + (1.f - t), pos.z + 0.f}; }
+        if (et & 16)   { float t = interp_lin(s[4], s[5]); vt[4]  = V3{pos.x + t, pos.y + 0.f, pos.z + 1.f}; }
+        if (et & 32)   { float t = interp_lin(s[5], s[6]); vt[5]  = V3{pos.x + 1.f, pos.y + t, pos.z + 1.f}; }
+        if (et & 64)   { float t = interp_lin(s[6], s[7]); vt[6]  = V3{pos.x + (1.f - t), pos.y + 1.f, pos.z + 1.f}; }
+        if (et & 128)  { float t = interp_lin(s[7], s[4]); vt[7]  = V3{pos.x + 0.f, pos.y + (1.f - t), pos.z + 1.f}; }
+        if (et & 256)  { float t = interp_lin(s[0], s[4]); vt[8]  = V3{pos.x + 0.f, pos.y + 0.f, pos.z + t}; }
+        if (et & 512)  { float t = interp_lin(s[1], s[5]); vt[9]  = V3{pos.x + 1.f, pos.y + 0.f, pos.z + t}; }
+        if (et & 1024) { float t = interp_lin(s[2], s[6]); vt[10] = V3{pos.x + 1.f, pos.y + 1.f, pos.z + t}; }
+        if (et & 2048) { float t = interp_lin(s[3], s[7]); vt[11] = V3{pos.x + 0.f, pos.y + 1.f, pos.z + t}; }
+        /* emit :225-240: triangle (a,b,c) is stored as (c,b,a) */
+        for (int i = 0; g_tri_table[cube][i] != -1; i += 3) {
+            int64_t t = first + count++;
+            if (t >= cap) continue;
+            V3 a = vt[g_tri_table[cube][i]], b = vt[g_tri_table[cube][i + 1]],
+               c = vt[g_tri_table[cube][i + 2]];
+            V3 p0{c.x * vs, c.y * vs, c.z * vs};
+            V3 p1{b.x * vs, b.y * vs, b.z * vs};
+            V3 p2{a.x * vs, a.y * vs, a.z * vs};
+            V3 n = normalized(cross(sub(p1, p0), sub(p2, p0)));
+            float* o = verts + t * 9;
+            o[0] = p0.x; o[1] = p0.y; o[2] = p0.z;
+            o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
+            o[6] = p2.x; o[7] = p2.y; o[8] = p2.z;
+            float* q = norms + t * 9;
+            for (int j = 0; j < 3; ++j) { q[j * 3] = n.x; q[j * 3 + 1] = n.y; q[j * 3 + 2] = n.z; }
+        }
+    }
+    return count;
+}
+
+/* Polygonize over every allocated entry in table order (select_blocks
+ * :25-50). The reference appends triangles through an atomic counter; here
+ * the order is table order, block by block, which a count pass, a prefix sum
+ * and a write pass reproduce with the blocks spread over threads. */
 int64_t or_polygonize(or_map* m, float* verts, float* norms, int64_t cap, int* block_count) {
     std::vector<int> blocks;
     for (int i = 0; i < m->nEntry; ++i)
-        if (m->table[i].ptr >= 0) blocks.push_back(i); /* select_blocks :25-50 */
+        if (m->table[i].ptr >= 0) blocks.push_back(i);
     if (block_count) *block_count = (int)blocks.size();
-    const float vs = m->voxelSize;
-    int64_t count = 0;
-    for (int bi : blocks) {
-        const Entry e = m->table[bi];
-        I3 base{e.x * kBlock, e.y * kBlock, e.z * kBlock};
-        for (int lz = 0; lz < kBlock; ++lz)
-        for (int ly = 0; ly < kBlock; ++ly)
-        for (int lx = 0; lx < kBlock; ++lx) {
-            V3 pos{(float)(base.x + lx), (float)(base.y + ly), (float)(base.z + lz)};
-            /* read_sdf_list :68-104, corners in Bourke order */
-            static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
-                                          {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-            float s[8];
-            bool ok = true;
-            for (int c = 0; c < 8 && ok; ++c) {
-                bool v;
-                s[c] = read_sdf(m, V3{pos.x + (float)off[c][0], pos.y + (float)off[c][1],
-                                      pos.z + (float)off[c][2]}, &v);
-                ok = v;
-            }
-            if (!ok) continue;
-            int cube = 0;
-            for (int c = 0; c < 8; ++c) if (s[c] < 0) cube |= 1 << c;
-            int et = g_edge_table[cube];
-            if (et == 0) continue;
-            V3 vt[12];
-            /* make_vertex :117-207 */
-            if (et & 1)    { float t = interp_lin(s[0], s[1]); vt[0]  = V3{pos.x + t, pos.y + 0.f, pos.z + 0.f}; }
-            if (et & 2)    { float t = interp_lin(s[1], s[2]); vt[1]  = V3{pos.x + 1.f, pos.y + t, pos.z + 0.f}; }
-            if (et & 4)    { float t = interp_lin(s[2], s[3]); vt[2]  = V3{pos.x + (1.f - t), pos.y + 1.f, pos.z + 0.f}; }
-            if (et & 8)    { float t = interp_lin(s[3], s[0]); vt[3]  = V3{pos.x + 0.f, pos.y + (1.f - t), pos.z + 0.f}; }
-            if (et & 16)   { float t = interp_lin(s[4], s[5]); vt[4]  = V3{pos.x + t, pos.y + 0.f, pos.z + 1.f}; }
-            if (et & 32)   { float t = interp_lin(s[5], s[6]); vt[5]  = V3{pos.x + 1.f, pos.y + t, pos.z + 1.f}; }
-            if (et & 64)   { float t = interp_lin(s[6], s[7]); vt[6]  = V3{pos.x + (1.f - t), pos.y + 1.f, pos.z + 1.f}; }
-            if (et & 128)  { float t = interp_lin(s[7], s[4]); vt[7]  = V3{pos.x + 0.f, pos.y + (1.f - t), pos.z + 1.f}; }
-            if (et & 256)  { float t = interp_lin(s[0], s[4]); vt[8]  = V3{pos.x + 0.f, pos.y + 0.f, pos.z + t}; }
-            if (et & 512)  { float t = interp_lin(s[1], s[5]); vt[9]  = V3{pos.x + 1.f, pos.y + 0.f, pos.z + t}; }
-            if (et & 1024) { float t = interp_lin(s[2], s[6]); vt[10] = V3{pos.x + 1.f, pos.y + 1.f, pos.z + t}; }
-            if (et & 2048) { float t = interp_lin(s[3], s[7]); vt[11] = V3{pos.x + 0.f, pos.y + 1.f, pos.z + t}; }
-            /* emit :225-240: triangle (a,b,c) is stored as (c,b,a) */
-            for (int i = 0; g_tri_table[cube][i] != -1; i += 3) {
-                int64_t t = count++;
-                if (t >= cap) continue;
-                V3 a = vt[g_tri_table[cube][i]], b = vt[g_tri_table[cube][i + 1]],
-                   c = vt[g_tri_table[cube][i + 2]];
-                V3 p0{c.x * vs, c.y * vs, c.z * vs};
-                V3 p1{b.x * vs, b.y * vs, b.z * vs};
-                V3 p2{a.x * vs, a.y * vs, a.z * vs};
-                V3 n = normalized(cross(sub(p1, p0), sub(p2, p0)));
-                float* o = verts + t * 9;
-                o[0] = p0.x; o[1] = p0.y; o[2] = p0.z;
-                o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
-                o[6] = p2.x; o[7] = p2.y; o[8] = p2.z;
-                float* q = norms + t * 9;
-                for (int j = 0; j < 3; ++j) { q[j * 3] = n.x; q[j * 3 + 1] = n.y; q[j * 3 + 2] = n.z; }
-            }
-        }
-    }
-    return std::min(count, cap);
+    const int64_t nb = (int64_t)blocks.size();
+    std::vector<int64_t> first(nb + 1, 0);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads)
+    for (int64_t k = 0; k < nb; ++k) first[k + 1] = mc_block(m, m->table[blocks[k]], nullptr, nullptr, 0, 0);
+    for (int64_t k = 0; k < nb; ++k) first[k + 1] += first[k];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads)
+    for (int64_t k = 0; k < nb; ++k)
+        if (first[k] < cap && first[k + 1] > first[k])
+            mc_block(m, m->table[blocks[k]], verts, norms, first[k], cap);
+    return std::min(first[nb], cap);
 }
 
 /* compute_surface_point_kernel, marching_cubes.cu:314-328 */

@@ -103,3 +103,80 @@ def canonical_triangles(v):
     v = np.asarray(v, np.float32).reshape(-1, 9)
     order = np.lexsort(v.T[::-1])
     return v[order], order
+
+
+def compare_map_blocks(g, o, fmt, sdf_tol=1e-5, check_chains=20000):
+    """compare_maps reading only what is allocated: the GPU's blocks [0, A)
+    (vm_map_download_blocks) and the oracle's pool in place, so a 2^20-block
+    pool is not copied whole. Same bar: key sets bit-exact, TSDF within
+    sdf_tol, weights and colours exact. Returns the number of blocks."""
+    gt, ot = g.hash_table(), o.hash_table()
+    gb, ob = block_map(gt), block_map(ot)
+    assert set(gb) == set(ob), (f"allocated sets differ: gpu-only {len(set(gb) - set(ob))}, "
+                                f"oracle-only {len(set(ob) - set(gb))}")
+    if check_chains:
+        chain_lookup_all(gt, g.desc.n_bucket, list(gb)[:check_chains])
+    if not gb:
+        return 0
+    keys = sorted(gb)
+    A = g.stats()["allocated_blocks"]
+    vb = {0: 4, 1: 6, 2: 12}[fmt]
+    graw = g.download_blocks(0, A).reshape(A, 512 * vb)
+    oraw = o.voxels_view().reshape(-1, 512 * vb)
+    gsel = graw[np.array([gb[k] // 512 for k in keys])].reshape(-1)
+    osel = oraw[np.array([ob[k] // 512 for k in keys])].reshape(-1)
+    gs, gw, gc = O.decode_voxels(gsel, fmt)
+    os_, ow, oc = O.decode_voxels(osel, fmt)
+    assert np.array_equal(gw, ow), f"weights differ at {np.sum(gw != ow)} voxels"
+    diff = np.abs(gs - os_)
+    assert diff.max() <= sdf_tol, f"TSDF differs by {diff.max()}"
+    if gc is not None:
+        assert np.array_equal(gc, oc), "colours differ"
+    return len(keys)
+
+
+def _soup_hash(v, n):
+    """64-bit hash of each triangle's 18 floats (bit patterns), wrapping."""
+    rows = np.concatenate([np.asarray(v, np.float32).reshape(-1, 9), np.asarray(n, np.float32).reshape(-1, 9)],
+                          axis=1).view(np.uint32).astype(np.uint64)
+    mult = (np.arange(1, 19, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)) | np.uint64(1)
+    h = np.zeros(len(rows), np.uint64)
+    for j in range(18):
+        h = (h ^ (rows[:, j] * mult[j])) * np.uint64(0xBF58476D1CE4E5B9)
+    return np.sort(h)
+
+
+def compare_soups(gv, gn, ov, on, tol=1e-5):
+    """Two triangle soups (vertices and face normals, 9 floats per triangle)
+    hold the same triangles in any order: first as multisets of exact bit
+    patterns (sorted row hashes); where that fails, after canonical sorting,
+    vertices and normals within tol."""
+    assert len(gv) == len(ov), f"triangle counts {len(gv)} vs {len(ov)}"
+    if len(gv) == 0 or np.array_equal(_soup_hash(gv, gn), _soup_hash(ov, on)):
+        return "bit-exact"
+    gs, gi = canonical_triangles(gv)
+    os_, oi = canonical_triangles(ov)
+    assert np.max(np.abs(gs - os_)) <= tol, f"vertices differ by {np.max(np.abs(gs - os_))}"
+    gnn = np.asarray(gn, np.float32).reshape(-1, 9)[gi]
+    onn = np.asarray(on, np.float32).reshape(-1, 9)[oi]
+    assert np.max(np.abs(gnn - onn)) <= 1e-4, f"normals differ by {np.max(np.abs(gnn - onn))}"
+    return "within tolerance"
+
+
+def compare_vmaps(gv, ov, tol=1e-4, min_hits=0.0):
+    """Hit masks equal, hit vertices within tol metres; returns the hit rate."""
+    gh, oh = ~np.isnan(gv[..., 0]), ~np.isnan(ov[..., 0])
+    assert np.array_equal(gh, oh), f"hit masks differ at {np.sum(gh != oh)} pixels"
+    d = np.abs(gv[gh][:, :3] - ov[oh][:, :3])
+    assert d.size == 0 or d.max() <= tol, f"vertex maps differ by {d.max()}"
+    assert gh.mean() >= min_hits, f"hit rate {gh.mean()}"
+    return gh.mean()
+
+
+def compare_normals(gn, on, tol=1e-4):
+    """Normal maps: NaN in the same places, finite values within tol (the
+    last row and column, never written, are left out)."""
+    gn, on = gn[:-1, :-1], on[:-1, :-1]
+    assert np.array_equal(np.isnan(gn), np.isnan(on))
+    fin = ~np.isnan(on)
+    assert not fin.any() or np.max(np.abs(gn[fin] - on[fin])) <= tol

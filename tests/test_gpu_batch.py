@@ -207,8 +207,16 @@ def test_batch_rewind_after_reset():
     for i, w in enumerate(ws):
         m, vm, nm = plain_run(w, range(3))
         same_maps(maps[i], m)
-        same_images(b.vmaps[i], vm)
-        same_images(b.nmaps[i][:-1, :-1], nm[:-1, :-1])
-        hits = (~torch.isnan(b.vmaps[i][..., 0])).float().mean().item()
-        assert hits > 0.3, f"scene {i}: hit rate {hits} after the rewind"
+        # a miss writes only x = NaN (as the reference), so the batched maps
+        # keep the y, z, w of the steps before the rewind there: compare the
+        # hit masks and every hit pixel
+        x, y = b.vmaps[i].cpu().numpy(), vm.cpu().numpy()
+        hit = ~np.isnan(x[..., 0])
+        assert np.array_equal(hit, ~np.isnan(y[..., 0]))
+        assert np.array_equal(x[hit], y[hit])
+        # normals of interior pixels whose four stencil neighbours are hits
+        nx, ny = b.nmaps[i].cpu().numpy()[1:-1, 1:-1], nm.cpu().numpy()[1:-1, 1:-1]
+        full = hit[1:-1, 1:-1] & hit[1:-1, :-2] & hit[1:-1, 2:] & hit[:-2, 1:-1] & hit[2:, 1:-1]
+        assert np.array_equal(nx[full], ny[full])
+        assert hit.mean() > 0.3, f"scene {i}: hit rate {hit.mean()} after the rewind"
     b.close()

@@ -11,6 +11,9 @@ The workload definitions used by the tests and the benchmark live here:
   seeded random walk (<= 2 cm, <= 1 degree per frame), 5 mm voxels, 2 cm
   truncation, 2^20 buckets + 2^17 excess slots, 2^20-block heap, float TSDF,
   float weight and uint8 colour.
+* :func:`batched16` -- 16 independent room_vga-style scenes of uneven image
+  size (160x120, 320x240, 640x480) and voxel size (4, 6, 10 mm), 2^16..2^20
+  buckets, for the multi-scene batch.
 """
 from __future__ import annotations
 
@@ -161,3 +164,22 @@ def room_vga(seed: int = 2, **kw) -> Workload:
                 voxel_format=2)
     args.update(kw)
     return Workload(**args)
+
+
+def batched16(n: int = 16, seed: int = 4) -> list:
+    """n seeded scenes: resolution from {160x120, 320x240, 640x480}, voxel
+    size from {4, 6, 10} mm (truncation 4 voxels), 2^16..2^20 buckets by image
+    size and voxel size, uint16 mm depth, float TSDF, random-walk trajectory."""
+    rng = np.random.default_rng(seed)
+    res = [(160, 120), (320, 240), (640, 480)]
+    out = []
+    for i in range(n):
+        W, H = res[rng.integers(3)]
+        vs = [0.004, 0.006, 0.010][rng.integers(3)]
+        f = 525.0 * W / 640
+        lb = {160: 16, 320: 18, 640: 20}[W] - (1 if vs >= 0.010 else 0) + (1 if vs <= 0.004 else 0)
+        lb = max(16, min(20, lb))
+        out.append(room_vga(seed=100 + i, width=W, height=H, K=(f, f, W / 2 - 0.5, H / 2 - 0.5),
+                            voxel_size=vs, trunc=4 * vs, n_bucket=1 << lb, n_excess=1 << (lb - 3),
+                            n_block=1 << min(lb, 18), voxel_format=2, name=f"scene{i}_{W}x{H}_{vs * 1000:.0f}mm"))
+    return out
