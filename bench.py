@@ -66,31 +66,46 @@ def frame_bytes(P, V):
     return P * (B_DEPTH + B_COLOR + B_VMAP + B_NMAP) + V * (512 * (2 * S_VOXEL + S_RAY) + S_ENTRY)
 
 
-def cpu_baseline(w, seconds: float) -> dict:
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, seconds: float, max_frames: int = 64) -> dict:
     """The oracle (OpenMP over blocks and pixels, serial allocation) on the
-    first frames of the same sequence, until `seconds` of work are done."""
+    first frames of the same sequence -- frame i fused at pose i, then the
+    raycast and normals at pose i, exactly the GPU step -- until `seconds` of
+    work or max_frames frames are done. Threads: the host's CPU share of one
+    GPU, at most 16 (what a one-GPU job may use on a shared box), below
+    that the cores this process may run on."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1))
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, avail))
     O.set_threads(threads)
     o = O.OracleMap(w.n_entry, w.n_bucket, w.n_block, w.voxel_size, w.trunc, w.voxel_format)
-    frames = [w.frame(i) for i in range(8)]
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        d, c = frames[n % len(frames)]
-        T = w.poses(n + 1)[n]
-        o.fuse(w.depth_metres(d), w.K, T, rgb=c)
-        vm = np.zeros((w.height, w.width, 4), np.float32)
-        o.render_scene(vm, w.K, T)
+    poses = w.poses(max_frames)
+    vm = np.zeros((w.height, w.width, 4), np.float32)
+    el, n = 0.0, 0
+    while n < max_frames and el < seconds:
+        d, c = w.frame(n)  # input generation is outside the timed work
+        t0 = time.perf_counter()
+        o.fuse(w.depth_metres(d), w.K, poses[n], rgb=c)
+        o.render_scene(vm, w.K, poses[n])
         O.compute_normals(vm)
+        el += time.perf_counter() - t0
         n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 64:
-            break
     return {"value": round(n / el, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"oracle fuse+raycast+normals of the first {n} frames of {w.name} "
-                      f"({el:.1f} s, OpenMP integration/raycast, serial allocation)"}
+            "nproc": os.cpu_count(), "affinity_cpus": avail, "cpu_model": _cpu_model(),
+            "sample": f"oracle fuse+raycast+normals of frames 0-{n - 1} of {w.name}, frame i at pose i "
+                      f"({el:.1f} s of work; {threads} OpenMP threads for integration, visibility and "
+                      f"raycast, serial allocation; g++ -O2)"}
 
 
 def batched_bench(warmup: int, steps: int) -> dict:
