@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle_vm.so")
+# VM_ORACLE_LIB points the tests' mutant check at a copy of the oracle built
+# with a planted mistake (tests/test_oracle_mutants.py)
+LIB_PATH = os.environ.get("VM_ORACLE_LIB") or os.path.join(HERE, "liboracle_vm.so")
 _lib = None
 
 _P = C.c_void_p

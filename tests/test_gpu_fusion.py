@@ -198,3 +198,25 @@ def test_invalid_arguments_raise():
         g.fuse(depth, (0.0, 1.0, 0.0, 0.0), np.eye(4))  # zero focal length
     with pytest.raises(VMError):
         g.fuse(depth, (1.0, 1.0, 0.0, 0.0), np.zeros((4, 4)))  # singular pose
+
+
+def test_visibility_border_and_depth_limits_on_gpu():
+    """The oracle's exact border cases of checkBlockVisible
+    (tests/test_oracle.py VIS_EDGE_CASES: corners projecting exactly onto
+    u = 0, u = cols - 1, v = 0, v = rows - 1, z = 0.1 m, z = 5 m, and half a
+    pixel or a block beyond): the GPU visibility pass decides each block as
+    the oracle does."""
+    from test_oracle import VIS_EDGE_CASES, one_block_visible
+    from voxelmapping_amd import DeviceMap
+    for block, vs, K, W, H, expect in VIS_EDGE_CASES:
+        o = O.OracleMap(64 + 16, 64, 16, vs, 4 * vs, 0)
+        assert o.insert_block(*block) >= 0
+        g = DeviceMap(64 + 16, 64, 16, vs, 4 * vs, 0)
+        g.upload(1, o.heap())
+        c = np.zeros(16, np.int32)
+        c[0], c[1] = o.counters()["heap_ptr"], o.counters()["excess_ptr"]
+        g.upload(3, c)
+        g.upload(0, o.hash_table())
+        n = g.check_visible(W, H, np.asarray(K, np.float32), np.eye(4, dtype=np.float32).reshape(-1))
+        assert (n == 1) == expect == one_block_visible(block, vs, K, W, H), (block, vs, K)
+        g.release()

@@ -182,6 +182,16 @@ def test_allocation_crosses_block_boundary():
     assert ptr[(0, 0, 12)] == 0 and ptr[(0, 0, 13)] == 512
 
 
+def test_allocation_half_truncation_band():
+    """The traced segment is depth -/+ HALF the truncation (depth_fusion.cu:
+    43-44, truncation * 0.5 at :226): at 0.9851 m the band [0.9651, 1.0051]
+    starts at voxel 96 (block 12); the full truncation would start it at
+    voxel 94 and add block 11."""
+    m = one_pixel_map()
+    m.allocate(np.array([[0.9851]], np.float32), K1, colmajor(np.eye(4)))
+    assert allocated(m) == {(0, 0, 12)}
+
+
 def test_allocation_negative_coordinates():
     m = one_pixel_map()
     flip = np.diag([1.0, -1.0, -1.0, 1.0])  # camera looks down world -z
@@ -542,3 +552,118 @@ def test_render_shading_closed_form():
         px = img[0, i + 1]
         assert px[3] == 255 and px[0] == px[1] == px[2], px
         assert abs(int(px[0]) - want) <= 1, (i, px, want)
+
+
+# --- raycast: sphere closed form ----------------------------------------------------
+#
+# A sphere of radius 0.25 m about 1 m in front of the camera, its exact
+# signed distance written into every voxel of the blocks around it
+# (clip(d / trunc, -1, 1), weight 1) at 5 mm voxels and 2 cm truncation.
+# Trilinear interpolation of that field moves the zero level by at most
+# h^2 / (8 r) = 1.25e-5 m. The march's two corrections after the first
+# sample with sdf <= 0 (ray_tracing.cu:381-393) each step by the normal
+# distance, which leaves (1 - cos t)^2 of the overshoot on a ray meeting the
+# surface at incidence t: below 1e-4 m for cos t >= 0.9, where one
+# correction alone leaves up to 10 %.
+
+SPH_C = np.array([0.0311, -0.0187, 1.0093])
+SPH_R, SPH_VS, SPH_TR = 0.25, 0.005, 0.02
+SPH_K = np.array([200.0, 200.0, 79.5, 63.5], np.float32)
+SPH_W, SPH_H = 160, 128
+
+
+def sphere_raycast_map():
+    m = O.OracleMap(16384 + 8192, 16384, 8192, SPH_VS, SPH_TR, 2)
+    rec = m.voxels_view().reshape(-1, 12)
+    lo = np.floor((SPH_C - SPH_R - 2 * SPH_TR) / SPH_VS).astype(int) // 8
+    hi = np.floor((SPH_C + SPH_R + 2 * SPH_TR) / SPH_VS).astype(int) // 8
+    lz, ly, lx = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    loc = np.stack([lx, ly, lz], -1).reshape(-1, 3)
+    one = np.frombuffer(np.float32(1).tobytes(), np.uint8)
+    for bx in range(lo[0], hi[0] + 1):
+        for by in range(lo[1], hi[1] + 1):
+            for bz in range(lo[2], hi[2] + 1):
+                ptr = m.insert_block(bx, by, bz)
+                assert ptr >= 0
+                p = (np.array([bx, by, bz]) * 8 + loc) * SPH_VS
+                s = np.clip((np.linalg.norm(p - SPH_C, axis=1) - SPH_R) / SPH_TR, -1, 1).astype(np.float32)
+                idx = ptr + (loc[:, 2] * 64 + loc[:, 1] * 8 + loc[:, 0])
+                rec[idx, 0:4] = s.view(np.uint8).reshape(-1, 4)
+                rec[idx, 4:8] = one
+    return m
+
+
+def sphere_rays():
+    """Per pixel: unit ray direction, analytic hit point (NaN on a miss),
+    cosine of the incidence angle, and the closest approach of the ray to the
+    centre."""
+    ys, xs = np.mgrid[0:SPH_H, 0:SPH_W]
+    d = np.stack([(xs - SPH_K[2]) / SPH_K[0], (ys - SPH_K[3]) / SPH_K[1], np.ones_like(xs, float)], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    b = d @ SPH_C
+    disc = b * b - (SPH_C @ SPH_C - SPH_R ** 2)
+    t = b - np.sqrt(np.maximum(disc, 0))
+    P = t[..., None] * d
+    nrm = (P - SPH_C) / SPH_R
+    cos = np.abs((nrm * d).sum(-1))
+    P[disc < 0] = np.nan
+    approach = np.sqrt(np.maximum(SPH_C @ SPH_C - b * b, 0))
+    return P, cos, approach
+
+
+def test_raycast_sphere_depth():
+    m = sphere_raycast_map()
+    v = m.raycast(SPH_W, SPH_H, SPH_K, colmajor(np.eye(4)))
+    P, cos, approach = sphere_rays()
+    hit = ~np.isnan(v[..., 0])
+    sel = cos >= 0.9
+    assert sel.sum() > 1000 and (sel & (cos < 0.97)).sum() > 100
+    assert hit[sel].all()
+    err = np.abs(v[..., :3][sel] - P[sel]).max()
+    assert err <= 1e-4, f"hit points off the sphere by {err} m"
+    # rays passing at least 2 voxels outside the sphere miss
+    away = approach > SPH_R + 2 * SPH_VS
+    assert away.sum() > 1000 and not hit[away].any()
+    # the per-pixel march (used by the adjoint's finite differences) agrees
+    ys, xs = np.mgrid[0:SPH_H, 0:SPH_W]
+    pv, _, ph = m.raycast_pixels_f32(xs.ravel(), ys.ravel(), SPH_K, colmajor(np.eye(4)))
+    assert np.array_equal(ph, hit.ravel()) and np.array_equal(pv[ph], v.reshape(-1, 4)[ph])
+
+
+# --- visibility at the image border and the depth limits ------------------------------
+#
+# checkBlockVisible (map_struct.h:204-244) keeps a block when any corner
+# projects to u in [0, cols - 1], v in [0, rows - 1] with z in [0.1, 5],
+# borders included. With an identity pose, power-of-two voxel sizes and
+# focal length 8, every corner projects exactly; each block below has only
+# one group of corners on the tested border line and every other corner
+# outside, so moving the line half a pixel (or using < for <=) flips it.
+
+def one_block_visible(block, vs, K, W, H):
+    m = O.OracleMap(64 + 16, 64, 16, vs, 4 * vs, 0)
+    assert m.insert_block(*block) >= 0
+    n = m.check_visible(W, H, np.asarray(K, np.float32), colmajor(np.eye(4)))
+    keys = {tuple(int(v) for v in e[2:]) for e in m.visible_table()}
+    assert n == len(keys) and keys <= {tuple(block)}
+    return n == 1
+
+
+VIS_EDGE_CASES = [
+    # block, voxel size, K, W, H, visible
+    ((16, 0, 7), 1 / 64, (8, 8, 15, 11.5), 32, 24, True),      # min u of the block = 31 = cols - 1
+    ((16, 0, 7), 1 / 64, (8, 8, 15.5, 11.5), 32, 24, False),   # min u = 31.5
+    ((-17, 0, 7), 1 / 64, (8, 8, 16, 11.5), 32, 24, True),     # max u = 0
+    ((-17, 0, 7), 1 / 64, (8, 8, 15.5, 11.5), 32, 24, False),  # max u = -0.5
+    ((0, 12, 7), 1 / 64, (8, 8, 15, 11), 32, 24, True),        # min v = 23 = rows - 1
+    ((0, 12, 7), 1 / 64, (8, 8, 15, 11.25), 32, 24, False),    # min v = 23.25
+    ((0, -13, 7), 1 / 64, (8, 8, 15, 12), 32, 24, True),       # max v = 0
+    ((0, 0, 0), 0.0125, (8, 8, 15, 11.5), 32, 24, True),       # nearest in-view corners at z = 0.1 exactly
+    ((0, 0, 8), 0.078125, (8, 8, 15, 11.5), 32, 24, True),     # nearest corners at z = 5.0 exactly
+    ((0, 0, 9), 0.078125, (8, 8, 15, 11.5), 32, 24, False),    # z in [5.625, 6.25]
+]
+
+
+@pytest.mark.parametrize("case", range(len(VIS_EDGE_CASES)))
+def test_visibility_border_and_depth_limits(case):
+    block, vs, K, W, H, expect = VIS_EDGE_CASES[case]
+    assert one_block_visible(block, vs, K, W, H) == expect
