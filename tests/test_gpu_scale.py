@@ -97,12 +97,11 @@ def test_two_frame_mesh_matches_oracle(large):
     torch.cuda.empty_cache()
 
 
-def _vertex_keys(t, vs):
-    """Key of the grid edge each marching-cubes vertex lies on: its lower
-    voxel corner and axis (the coordinate off the integer grid); a vertex at
-    a corner keys as the corner. Neighbouring cubes compute a shared edge's
-    vertex from opposite ends, so its floats may differ in the last bit,
-    but never its edge."""
+def _vertex_grid(t, vs):
+    """The grid edge each marching-cubes vertex lies on: its lower voxel
+    corner and axis (the coordinate off the integer grid; 3 for a vertex at
+    a corner). Neighbouring cubes compute a shared edge's vertex from
+    opposite ends, so its floats may differ in the last bit, never its edge."""
     u = t.astype(np.float64) / vs
     r = np.round(u)
     fr = np.abs(u - r)
@@ -111,28 +110,37 @@ def _vertex_keys(t, vs):
     lo = r.copy()
     ax_u = np.take_along_axis(u, axis[..., None], -1)[..., 0]
     np.put_along_axis(lo, axis[..., None], np.floor(ax_u)[..., None], -1)
-    axis = np.where(corner, 3, axis)
-    lo = lo.astype(np.int64)
-    return ((lo[..., 0] * 73856093) ^ (lo[..., 1] * 19349669) ^ (lo[..., 2] * 83492791)) * 4 + axis
+    return lo.astype(np.int64), np.where(corner, 3, axis)
 
 
-def _open_edge_fraction(v, vs, lo, hi):
-    """Among triangles with all three vertices in the box [lo, hi) (metres),
-    the fraction of their edges not shared by exactly two triangles."""
+def _open_edges_by_plane(v, vs, lo_b, hi_b):
+    """Triangles with all vertices in the box [lo_b, hi_b) (metres): of their
+    edges lying in a grid plane at a block boundary (voxel coordinate 0 or 7
+    mod 8) and of those in a plane inside a block (3 or 4 mod 8), the
+    fractions not shared by exactly two triangles. A block whose halo or
+    neighbour layer is missing opens the surface along block boundaries
+    only; noise and unobserved voxels open it anywhere."""
     t = v.reshape(-1, 3, 3)
-    near = np.all((t >= lo - 2 * vs) & (t < hi + 2 * vs), axis=(1, 2))
+    near = np.all((t >= lo_b - 2 * vs) & (t < hi_b + 2 * vs), axis=(1, 2))
     t = t[near]
-    inside = np.all((t >= lo) & (t < hi), axis=(1, 2))
-    key = _vertex_keys(t, vs)
-    e = []
+    inside = np.all((t >= lo_b) & (t < hi_b), axis=(1, 2))
+    lo, ax = _vertex_grid(t, vs)
+    key = ((lo[..., 0] * 73856093) ^ (lo[..., 1] * 19349669) ^ (lo[..., 2] * 83492791)) * 4 + ax
+    e, plane = [], []
     for a, b in ((0, 1), (1, 2), (2, 0)):
         ka, kb = key[:, a], key[:, b]
         e.append(np.where(ka < kb, ka * 1000003 + kb, kb * 1000003 + ka))
-    e = np.stack(e, axis=1)
+        f = np.zeros(len(ka), np.int8)  # 1: block-boundary plane, 2: inner plane
+        for k in range(3):
+            on = (ax[:, a] != k) & (ax[:, b] != k) & (lo[:, a, k] == lo[:, b, k])
+            md = lo[:, a, k] % 8
+            f = np.where(on & ((md == 0) | (md == 7)), 1, np.where(on & ((md == 3) | (md == 4)) & (f == 0), 2, f))
+        plane.append(f)
+    e, plane = np.stack(e, axis=1), np.stack(plane, axis=1)
     uniq, cnt = np.unique(e.reshape(-1), return_counts=True)
-    mine = e[inside].reshape(-1)
-    c = cnt[np.searchsorted(uniq, mine)]
-    return (float(np.mean(c != 2)) if len(mine) else 0.0), int(inside.sum())
+    mine, pm = e[inside].reshape(-1), plane[inside].reshape(-1)
+    opened = cnt[np.searchsorted(uniq, mine)] != 2
+    return float(opened[pm == 1].mean()), float(opened[pm == 2].mean()), int(inside.sum())
 
 
 def test_forty_frames_invariants(large):
@@ -168,9 +176,11 @@ def test_forty_frames_invariants(large):
 def test_forty_frame_mesh(large):
     """The full mesh of the 40-frame map (runs after the forty frames are
     fused): two calls give identical buffers and counts, and inside sampled
-    boxes on the surface nearly every edge is shared by exactly two
-    triangles (a missing halo layer or neighbour block would leave whole
-    block faces open)."""
+    boxes on the surface the edges left open (shared by other than two
+    triangles) are no more frequent on block-boundary planes than inside
+    blocks. This workload's depth noise at 5 m (3 cm) is three times its
+    truncation, so its far walls have holes everywhere; a missing halo layer
+    or neighbour block would open them along block faces."""
     import torch
     w, g, _, _ = large
     st = g.stats()
@@ -192,14 +202,14 @@ def test_forty_frame_mesh(large):
     torch.cuda.empty_cache()
     assert np.all(np.isfinite(v))
     rng = np.random.default_rng(9)
-    fracs = []
+    excess = []
     for c in v.reshape(-1, 3)[rng.choice(gt * 3, 6, replace=False)]:
-        f, n = _open_edge_fraction(v, w.voxel_size, c - 0.15, c + 0.15)
-        fracs.append(f)
-        print(f"box at {c}: {n} triangles, open-edge fraction {f:.4f}")
-    # the oracle's meshes of the smaller rooms measure 0-0.7 % (holes where
-    # voxels were never observed, ambiguous faces of the classic table)
-    assert np.median(fracs) <= 0.01 and max(fracs) <= 0.05, fracs
+        fb, fi, n = _open_edges_by_plane(v, w.voxel_size, c - 0.15, c + 0.15)
+        excess.append(fb - 2 * fi)
+        print(f"box at {c}: {n} triangles, open edges {fb:.4f} on block-boundary planes, {fi:.4f} inside blocks")
+    # the oracle's meshes of the smaller rooms give -0.01..0.002 here; a
+    # missing layer along block boundaries gives 0.05..0.55
+    assert np.median(excess) <= 0.01, excess
 
 
 def test_pose_refinement_at_scale():
