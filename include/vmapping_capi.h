@@ -177,6 +177,20 @@ int vm_render_shading(const float* vmap, int64_t vmap_pitch, const float* nmap, 
  * block, kept until vm_map_destroy. */
 int vm_polygonize(vm_map* map, float* verts, float* norms, int64_t cap_triangles, void* stream,
                   uint32_t* out_block_count, uint32_t* out_triangle_count);
+/* vm_polygonize without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops
+ * and harnesses that forbid hipMalloc, set-up of captured work). Same results
+ * as vm_polygonize, into the caller's verts / norms; the mesh work buffer is
+ * a fixed array in the library (18 MiB per device, held until the process
+ * ends) instead of one allocated by the first call. Maps with more than
+ * VM_NOALLOC_MAX_BLOCKS allocated blocks are rejected (VM_ERR_INVALID_ARG).
+ * The map's device must be current. All calls on one device share that
+ * array: calls that overlap on one device (from other streams or threads)
+ * race. Synchronises `stream` before the launches (to check the block
+ * count) and after them. */
+#define VM_NOALLOC_MAX_BLOCKS (1 << 19)
+int vm_polygonize_noalloc(vm_map* map, float* verts, float* norms, int64_t cap_triangles, void* stream,
+                          uint32_t* out_block_count, uint32_t* out_triangle_count);
 /* GetSurfacePoints (marching_cubes.cu:330-344): triangle centroids. */
 int vm_surface_points(const float* verts, float* points, int64_t n_triangles, void* stream);
 

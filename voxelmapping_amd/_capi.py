@@ -135,6 +135,7 @@ SIGNATURES = {
     "vm_compute_normals": (C.c_int, [_P, C.c_int64, _P, C.c_int64, C.c_int32, C.c_int32, _P]),
     "vm_render_shading": (C.c_int, [_P, C.c_int64, _P, C.c_int64, _P, C.c_int64, C.c_int32, C.c_int32, _P]),
     "vm_polygonize": (C.c_int, [_P, _P, _P, C.c_int64, _P, _U32P, _U32P]),
+    "vm_polygonize_noalloc": (C.c_int, [_P, _P, _P, C.c_int64, _P, _U32P, _U32P]),
     "vm_surface_points": (C.c_int, [_P, _P, C.c_int64, _P]),
     "vm_raycast_backward": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, _P, C.c_int64, C.c_int32, C.c_int32, _F4,
                                       _F4, _P, _P, _P, C.POINTER(BackwardStats)]),

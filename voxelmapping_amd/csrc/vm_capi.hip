@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -198,6 +199,31 @@ void remember_visibility(vm_map* m, int W, int H, const float* K, const float* T
 bool visibility_current(const vm_map* m, int W, int H, const float* K, const float* T) {
     return m->visValid && m->visEpoch == m->epoch && m->visW == W && m->visH == H &&
            std::memcmp(m->visK, K, sizeof(m->visK)) == 0 && std::memcmp(m->visT, T, sizeof(m->visT)) == 0;
+}
+
+int check_polygonize_args(const vm_map* m, const float* verts, const float* norms, int64_t cap) {
+    if (check_map(m)) return VM_ERR_INVALID_ARG;
+    if (cap < 0) return fail(VM_ERR_INVALID_ARG, "cap_triangles must be >= 0");
+    if (cap > 0 && (!verts || !norms)) return fail(VM_ERR_INVALID_ARG, "verts and norms must not be NULL");
+    return VM_OK;
+}
+
+// After the mesh launches on s: wait, read the counters, flag a clamped
+// mesh, return the block count and the triangles kept.
+int polygonize_finish(vm_map* m, int64_t cap, hipStream_t s, uint32_t* out_blocks, uint32_t* out_tris) {
+    int c[kCounterInts];
+    VM_HIP(hipMemcpyAsync(c, m->view.ctr, sizeof(c), hipMemcpyDeviceToHost, s), "counter copy");
+    VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    const int64_t tris = (int64_t)((uint64_t)(uint32_t)c[kTriangleTotalLo] | ((uint64_t)(uint32_t)c[kTriangleTotalHi] << 32));
+    const int64_t kept = std::min<int64_t>(std::min<int64_t>(tris, cap), 0xffffffffll);
+    if (tris > kept) {
+        int st = c[kStatus] | kStatusMeshClamped;
+        VM_HIP(hipMemcpyAsync(m->view.ctr + kStatus, &st, sizeof(int), hipMemcpyHostToDevice, s), "status update");
+        VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    }
+    if (out_blocks) *out_blocks = (uint32_t)(m->view.nBlock - 1 - c[kHeapPtr]);
+    if (out_tris) *out_tris = (uint32_t)kept;
+    return VM_OK;
 }
 
 int read_counter(vm_map* m, int idx, hipStream_t s, int* out) {
@@ -506,26 +532,81 @@ int vm_render_shading(const float* vmap, int64_t vp, const float* nmap, int64_t 
 
 int vm_polygonize(vm_map* m, float* verts, float* norms, int64_t cap, void* stream, uint32_t* out_blocks,
                   uint32_t* out_tris) {
-    if (check_map(m)) return VM_ERR_INVALID_ARG;
-    if (cap < 0) return fail(VM_ERR_INVALID_ARG, "cap_triangles must be >= 0");
-    if (cap > 0 && (!verts || !norms)) return fail(VM_ERR_INVALID_ARG, "verts and norms must not be NULL");
+    if (int rc = check_polygonize_args(m, verts, norms, cap)) return rc;
     hipStream_t s = (hipStream_t)stream;
+    const int nb = m->view.nBlock;
     if (!m->meshWork)
-        VM_HIP(hipMalloc((void**)&m->meshWork, polygonize_work_ints(m->view.nBlock) * sizeof(int)),
+        VM_HIP(hipMalloc((void**)&m->meshWork, polygonize_work_ints(nb, polygonize_groups(nb)) * sizeof(int)),
                "mesh work allocation");
-    VM_HIP(launch_polygonize(m->view, m->meshWork, verts, norms, cap, s), "polygonize");
-    int c[kCounterInts];
-    VM_HIP(hipMemcpyAsync(c, m->view.ctr, sizeof(c), hipMemcpyDeviceToHost, s), "counter copy");
-    VM_HIP(hipStreamSynchronize(s), "stream synchronize");
-    const int64_t tris = (int64_t)((uint64_t)(uint32_t)c[kTriangleTotalLo] | ((uint64_t)(uint32_t)c[kTriangleTotalHi] << 32));
-    const int64_t kept = std::min<int64_t>(std::min<int64_t>(tris, cap), 0xffffffffll);
-    if (tris > kept) {
-        int st = c[kStatus] | kStatusMeshClamped;
-        VM_HIP(hipMemcpy(m->view.ctr + kStatus, &st, sizeof(int), hipMemcpyHostToDevice), "status update");
+    VM_HIP(launch_polygonize(m->view, m->meshWork, nb, verts, norms, cap, s), "polygonize");
+    return polygonize_finish(m, cap, s, out_blocks, out_tris);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// marching cubes without device allocation
+
+namespace {
+
+constexpr int kNoAllocMaxBlocks = VM_NOALLOC_MAX_BLOCKS;
+constexpr int kNoAllocMaxGroups = 4096;  // emitting workgroups (1536 on 256 compute units)
+constexpr size_t kNoAllocInts = 9 * (size_t)kNoAllocMaxBlocks + 1 + 2 * (size_t)kNoAllocMaxGroups;
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// The mesh work buffer of vm_polygonize_noalloc: 18 MiB per device that
+// loads the library, held until the process ends. Host code never uses
+// this name as an address: each device's copy is found with
+// hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+namespace {
+
+std::mutex g_scratchMutex;
+int* g_scratchPtr[kMaxDevices];
+
+// the address of this device's `scratch`, looked up once per device with
+// the device current
+int noalloc_scratch(int dev, int** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(VM_ERR_INVALID_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "scratch symbol address");
+        g_scratchPtr[dev] = (int*)p;
     }
-    if (out_blocks) *out_blocks = (uint32_t)(m->view.nBlock - 1 - c[kHeapPtr]);
-    if (out_tris) *out_tris = (uint32_t)kept;
+    *out = g_scratchPtr[dev];
     return VM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vm_polygonize_noalloc(vm_map* m, float* verts, float* norms, int64_t cap, void* stream, uint32_t* out_blocks,
+                          uint32_t* out_tris) {
+    if (int rc = check_polygonize_args(m, verts, norms, cap)) return rc;
+    int dev = -1;
+    VM_HIP(hipGetDevice(&dev), "current device");
+    if (dev != m->device) return fail(VM_ERR_INVALID_ARG, "the map's device must be the current device");
+    int* work = nullptr;
+    if (int rc = noalloc_scratch(dev, &work)) return rc;
+    if (polygonize_groups(m->view.nBlock) > kNoAllocMaxGroups)
+        return fail(VM_ERR_INVALID_ARG, "too many compute units for the fixed mesh work array");
+    hipStream_t s = (hipStream_t)stream;
+    // the allocated blocks must fit the fixed work array
+    int heapPtr = 0;
+    if (int rc = read_counter(m, kHeapPtr, s, &heapPtr)) return rc;
+    const int64_t A = (int64_t)m->view.nBlock - 1 - heapPtr;
+    if (A > kNoAllocMaxBlocks)
+        return fail(VM_ERR_INVALID_ARG, "vm_polygonize_noalloc: " + std::to_string(A) +
+                                            " allocated blocks exceed VM_NOALLOC_MAX_BLOCKS (" +
+                                            std::to_string(kNoAllocMaxBlocks) + "); use vm_polygonize");
+    VM_HIP(launch_polygonize(m->view, work, kNoAllocMaxBlocks, verts, norms, cap, s), "polygonize");
+    return polygonize_finish(m, cap, s, out_blocks, out_tris);
 }
 
 int vm_raycast_backward(vm_map* m, const float* vmap, int64_t vpitch, const float* gv, int64_t gvpitch,

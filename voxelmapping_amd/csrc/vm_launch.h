@@ -44,11 +44,15 @@ hipError_t launch_normals(const float* vmap, int64_t vpitch, float* nmap, int64_
 hipError_t launch_shading(const float* vmap, int64_t vpitch, const float* nmap, int64_t npitch,
                           uint8_t* rgba, int64_t rpitch, int width, int height, hipStream_t s);
 // BuildVertexArray / Polygonize, marching_cubes.cu:9-305; `work` holds
-// polygonize_work_ints(nBlock) ints (neighbour offsets, per-block triangle
-// counts, workgroup offsets; rewritten by every call)
-size_t polygonize_work_ints(int nBlock);
-hipError_t launch_polygonize(const MapView& m, int* work, float* verts, float* norms, int64_t capTriangles,
-                             hipStream_t s);
+// polygonize_work_ints(capBlocks, polygonize_groups(nBlock)) ints
+// (neighbour offsets, per-block triangle counts, workgroup offsets;
+// rewritten by every call, nothing in it needs clearing). The passes take
+// the first min(allocated, capBlocks) blocks: callers size capBlocks to the
+// pool or check the allocated count first.
+int polygonize_groups(int nBlock);
+size_t polygonize_work_ints(int capBlocks, int groups);
+hipError_t launch_polygonize(const MapView& m, int* work, int capBlocks, float* verts, float* norms,
+                             int64_t capTriangles, hipStream_t s);
 // compute_surface_point_kernel, marching_cubes.cu:314-344
 hipError_t launch_surface_points(const float* verts, float* points, int64_t n, hipStream_t s);
 

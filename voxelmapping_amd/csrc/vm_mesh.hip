@@ -33,12 +33,19 @@ constexpr int kMcThreads = 256;           // one workgroup (4 waves) per block
 constexpr int kMcMaxTri = kBlockVox * 5;  // at most 5 triangles per cube
 constexpr int kMcGroupsPerCu = 6;         // emit pass: ~23.7 KB of LDS per workgroup
 
+// Allocated blocks the mesh passes take: the allocated count, at most the
+// capacity of the work buffer (the caller checks it fits).
+__device__ inline int mesh_blocks(const int* ctr, int nBlock, int capBlocks) {
+    const int a = allocated_count(ctr, nBlock);
+    return a < capBlocks ? a : capBlocks;
+}
+
 // Voxel offsets of every allocated block and its 7 +x/+y/+z neighbours
 // (links[8 b + n], n = dx | dy << 1 | dz << 2; -1: absent; links[8 b] = -2:
 // dead block), one thread per (block, neighbour): the hash probes of the
 // mesh, all in flight at once.
-__global__ void k_mc_links(MapView m, int* links) {
-    const int A = allocated_count(m.ctr, m.nBlock);
+__global__ void k_mc_links(MapView m, int* links, int capBlocks) {
+    const int A = mesh_blocks(m.ctr, m.nBlock, capBlocks);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)A * 8;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int b = (int)(i >> 3), n = (int)(i & 7);
@@ -148,11 +155,11 @@ __device__ inline int mc_classify(McShared& sh, int tid, int nt[2], int& total) 
 // Counting pass: the triangle count of every block (blockTri[b]), over a
 // grid of its own.
 template <int FMT>
-__global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* links, int* blockTri) {
+__global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* links, int* blockTri, int capBlocks) {
     __shared__ McShared sh;
     const int tid = threadIdx.x;
     mc_load_tables(sh, tid);
-    const int A = allocated_count(m.ctr, m.nBlock);
+    const int A = mesh_blocks(m.ctr, m.nBlock, capBlocks);
     // per-lane (vector) loads: their wait is at the use, one block later
     int nbNext = -1;
     if ((int)blockIdx.x < A && tid < 8) nbNext = links[(int64_t)blockIdx.x * 8 + tid];
@@ -181,11 +188,11 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* l
 // exceed 2^32); a workgroup's own sum, at most 2560 per block of its share,
 // fits 32 bits.
 __global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, unsigned long long* wgBase, int* ctr,
-                                                  int nBlock) {
+                                                  int nBlock, int capBlocks) {
     __shared__ unsigned long long s_w[16];
     __shared__ unsigned long long s_carry;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int A = allocated_count(ctr, nBlock);
+    const int A = mesh_blocks(ctr, nBlock, capBlocks);
     if (tid == 0) s_carry = 0ull;
     __syncthreads();
     for (int c0 = 0; c0 < g; c0 += 1024) {
@@ -232,14 +239,14 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, un
 template <int FMT>
 __global__ __launch_bounds__(kMcThreads) void k_mc_emit(MapView m, const int* links, const int* blockTri,
                                                          const unsigned long long* wgBase, float* verts, float* norms,
-                                                         int64_t cap) {
+                                                         int64_t cap, int capBlocks) {
     __shared__ McShared sh;
     __shared__ uint16_t s_map[kMcMaxTri];  // triangle -> (cube << 3) | k
     __shared__ float s_v[kMcThreads * 9];
     __shared__ float s_n[kMcThreads * 3];
     const int tid = threadIdx.x;
     mc_load_tables(sh, tid);
-    const int A = allocated_count(m.ctr, m.nBlock);
+    const int A = mesh_blocks(m.ctr, m.nBlock, capBlocks);
     const float vs = m.voxelSize;
     int64_t base = (int64_t)wgBase[blockIdx.x];
     // per-lane (vector) loads: their wait is at the use, one block later
@@ -364,37 +371,44 @@ static int mc_groups(int nBlock, int perCu) {
     return g < nBlock ? g : (nBlock > 0 ? nBlock : 1);
 }
 
-// links (8 per block), blockTri (1 per block), then the emitting
-// workgroups' 64-bit offsets at an even int index
-size_t polygonize_work_ints(int nBlock) {
-    return (size_t)nBlock * 9 + 1 + 2 * (size_t)mc_groups(nBlock, kMcGroupsPerCu);
-}
+int polygonize_groups(int nBlock) { return mc_groups(nBlock, kMcGroupsPerCu); }
 
-hipError_t launch_polygonize(const MapView& m, int* work, float* verts, float* norms, int64_t cap, hipStream_t s) {
-    const int g = mc_groups(m.nBlock, kMcGroupsPerCu);
+// links (8 per block of capacity), blockTri (1 per block), then the emitting
+// workgroups' 64-bit offsets at an even int index
+size_t polygonize_work_ints(int capBlocks, int groups) { return (size_t)capBlocks * 9 + 1 + 2 * (size_t)groups; }
+
+hipError_t launch_polygonize(const MapView& m, int* work, int capBlocks, float* verts, float* norms, int64_t cap,
+                             hipStream_t s) {
+    const int g = polygonize_groups(m.nBlock);
     const int gc = mc_groups(m.nBlock, 8);  // counting pass: 36 VGPRs, ~6.3 KB of LDS
     int* links = work;
-    int* blockTri = work + (size_t)m.nBlock * 8;
-    unsigned long long* wgBase = (unsigned long long*)(work + (((size_t)m.nBlock * 9 + 1) & ~(size_t)1));
-    hipLaunchKernelGGL(k_mc_links, dim3(num_compute_units() * 8), dim3(256), 0, s, m, links);
+    int* blockTri = work + (size_t)capBlocks * 8;
+    unsigned long long* wgBase = (unsigned long long*)(work + (((size_t)capBlocks * 9 + 1) & ~(size_t)1));
+    hipLaunchKernelGGL(k_mc_links, dim3(num_compute_units() * 8), dim3(256), 0, s, m, links, capBlocks);
     const dim3 block(kMcThreads);
     switch (m.fmt) {
-        case kFmtRef: hipLaunchKernelGGL(k_mc_count<kFmtRef>, dim3(gc), block, 0, s, m, links, blockTri); break;
-        case kFmtRefRGB: hipLaunchKernelGGL(k_mc_count<kFmtRefRGB>, dim3(gc), block, 0, s, m, links, blockTri); break;
-        default: hipLaunchKernelGGL(k_mc_count<kFmtF32>, dim3(gc), block, 0, s, m, links, blockTri);
+        case kFmtRef:
+            hipLaunchKernelGGL(k_mc_count<kFmtRef>, dim3(gc), block, 0, s, m, links, blockTri, capBlocks);
+            break;
+        case kFmtRefRGB:
+            hipLaunchKernelGGL(k_mc_count<kFmtRefRGB>, dim3(gc), block, 0, s, m, links, blockTri, capBlocks);
+            break;
+        default: hipLaunchKernelGGL(k_mc_count<kFmtF32>, dim3(gc), block, 0, s, m, links, blockTri, capBlocks);
     }
-    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, blockTri, g, wgBase, m.ctr, m.nBlock);
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, blockTri, g, wgBase, m.ctr, m.nBlock, capBlocks);
     const dim3 grid(g);
     switch (m.fmt) {
         case kFmtRef:
-            hipLaunchKernelGGL(k_mc_emit<kFmtRef>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms, cap);
+            hipLaunchKernelGGL(k_mc_emit<kFmtRef>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms, cap,
+                               capBlocks);
             break;
         case kFmtRefRGB:
             hipLaunchKernelGGL(k_mc_emit<kFmtRefRGB>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms,
-                               cap);
+                               cap, capBlocks);
             break;
         default:
-            hipLaunchKernelGGL(k_mc_emit<kFmtF32>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms, cap);
+            hipLaunchKernelGGL(k_mc_emit<kFmtF32>, grid, block, 0, s, m, links, blockTri, wgBase, verts, norms, cap,
+                               capBlocks);
     }
     return hipGetLastError();
 }

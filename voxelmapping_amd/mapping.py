@@ -231,6 +231,18 @@ class DeviceMap:
                                   _stream(stream), C.byref(b), C.byref(t)))
         return int(b.value), int(t.value)
 
+    def polygonize_noalloc(self, verts, norms, stream=None):
+        """polygonize through vm_polygonize_noalloc: no device allocation, the
+        library's fixed mesh work array (at most VM_NOALLOC_MAX_BLOCKS
+        allocated blocks)."""
+        cap = verts.shape[0]
+        if norms.shape[0] != cap or verts.numel() != cap * 9 or norms.numel() != cap * 9:
+            raise ValueError("verts and norms must be (cap, 9) float32")
+        b, t = C.c_uint32(0), C.c_uint32(0)
+        check(lib().vm_polygonize_noalloc(self._h, C.c_void_p(verts.data_ptr()), C.c_void_p(norms.data_ptr()), cap,
+                                          _stream(stream), C.byref(b), C.byref(t)))
+        return int(b.value), int(t.value)
+
     # --- state transfer -------------------------------------------------------
     def _nbytes(self, which):
         d = self.desc
