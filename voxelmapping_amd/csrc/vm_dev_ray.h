@@ -327,11 +327,72 @@ __device__ inline void ray_begin(Ray& r, int x, int y, const Intrinsics& K, cons
 
 enum RayStatus : int { kRayGoing = 0, kRaySurface = 1, kRayMiss = 2 };
 
+// Runs of predictable samples (forward march only). A sample in an
+// allocated block that reads "invalid" (an unobserved voxel, or an
+// interpolation with an unobserved corner) steps exactly one voxel, and on
+// the rays that take longest it is mostly followed by more of them: rays
+// crossing the unobserved side of fused surfaces. Likewise a valid voxel of
+// exactly 1.0 (free space in front of a surface) steps rtStep. The
+// positions of such followers are known before their reads return, so the
+// next kRunSpec sample positions inside the same block are computed with the
+// loop's own additions, their voxels read together, and the prefix that
+// reads the same kind is taken (each such sample only steps); the march
+// resumes at the first sample that reads anything else, the block's end or
+// distEnd. One memory round trip then covers up to kRunSpec samples that the
+// loop reads one after the other. Positions, distances and results are the
+// loop's own, bit for bit. Which runs are taken is a template choice of
+// ray_step: kRunsNone, kRunsInvalid (unit steps), kRunsAll (also 1.0 steps).
+enum RunKinds : int { kRunsNone = 0, kRunsInvalid = 1, kRunsAll = 2 };
+constexpr int kRunSpec = 4;
+
+// r is just past a sample of voxel block (bx, by, bz) (pool offset p) that
+// read "invalid" (valid = false, step 1) or a valid 1.0 (step rtStep), and
+// took its step `step`.
+template <int FMT>
+__device__ inline void ray_run(Ray& r, const RayReader<FMT>& rd, int p, int bx, int by, int bz, float step,
+                               bool valid) {
+    int loc[kRunSpec];
+    int n = 0;
+    float3 q = r.res;
+    float d = r.dist_s;
+#pragma unroll
+    for (int k = 0; k < kRunSpec; ++k) {
+        loc[k] = 0;
+        if (n == k && d < r.distEnd) {  // the loop condition of sample k
+            const int qx = floor_i(q.x), qy = floor_i(q.y), qz = floor_i(q.z);
+            if (vox_to_blk(qx) == bx && vox_to_blk(qy) == by && vox_to_blk(qz) == bz) {
+                loc[k] = local_index(vox_to_loc(qx), vox_to_loc(qy), vox_to_loc(qz));
+                q = make_float3(q.x + step * r.dir.x, q.y + step * r.dir.y, q.z + step * r.dir.z);
+                d += step;
+                n = k + 1;
+            }
+        }
+    }
+    float sv[kRunSpec];
+    bool ok[kRunSpec];
+#pragma unroll
+    for (int k = 0; k < kRunSpec; ++k) {
+        sv[k] = 1.0f;
+        ok[k] = false;
+        if (k < n) Vox<FMT>::read(rd.m.voxels, (int64_t)p + loc[k], sv[k], ok[k]);
+    }
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < kRunSpec; ++k)
+        if (acc == k && k < n && (valid ? (ok[k] && sv[k] == 1.0f) : !ok[k])) acc = k + 1;
+    for (int k = 0; k < acc; ++k) {
+        r.res = make_float3(r.res.x + step * r.dir.x, r.res.y + step * r.dir.y, r.res.z + step * r.dir.z);
+        r.dist_s += step;
+    }
+}
+
 // One pass of the march loop body (ray_tracing.cu:355-379): a sample, or a
 // run of samples in empty blocks. kRaySurface: the loop broke on sdf <= 0;
 // kRayMiss: it returned on a sign change or ran out of distance.
-// rtStep = trunc / voxelSize (ray_tracing.cu:431).
-template <int FMT>
+// rtStep = trunc / voxelSize (ray_tracing.cu:431). RUNS (RunKinds): the
+// runs of samples taken after a sample that starts one (ray_run) -- for the
+// forward raycast only, as a run's samples are not visited one by one.
+template <int FMT, int RUNS = kRunsNone>
 __device__ inline int ray_step(Ray& r, RayReader<FMT>& rd, float rtStep) {
     if (!(r.dist_s < r.distEnd)) return kRayMiss;  // loop condition; sdf > 0 or NaN here
     const float last = r.sdf;
@@ -353,6 +414,8 @@ __device__ inline int ray_step(Ray& r, RayReader<FMT>& rd, float rtStep) {
     const float step = valid ? fmaxf(sdf * rtStep, 1.0f) : 1.0f;
     r.res = make_float3(r.res.x + step * r.dir.x, r.res.y + step * r.dir.y, r.res.z + step * r.dir.z);
     r.dist_s += step;
+    if (RUNS != kRunsNone && p >= 0 && (!valid || (RUNS == kRunsAll && sdf == 1.0f)))
+        ray_run(r, rd, p, bx, by, bz, step, valid);
     return kRayGoing;
 }
 
@@ -402,7 +465,7 @@ __device__ inline void ray_skip(Ray& r, float nearLim) {
 // before nearLim are skipped, and a ray that reaches farLim can only read
 // (1.0, invalid) from there to distEnd, which ends in a miss, so the march
 // stops there.
-template <int FMT>
+template <int FMT, int RUNS = kRunsNone>
 __device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y, const Intrinsics& K,
                                      const Affine& pose, const Affine& Tinv, float vsInv, float rtStep,
                                      float nearLim = 0.0f, float farLim = INFINITY) {
@@ -412,7 +475,7 @@ __device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y,
     ray_skip(r, nearLim);
     RayReader<FMT> rd(m);
     int st;
-    while ((st = ray_step(r, rd, rtStep)) == kRayGoing) {
+    while ((st = ray_step<FMT, RUNS>(r, rd, rtStep)) == kRayGoing) {
     }
     float4 v;
     if (st == kRaySurface && ray_surface(r, rd, Tinv, vsInv, rtStep, v)) *(float4*)out = v;

@@ -18,8 +18,14 @@ namespace vmamd {
 
 namespace {
 
+// the runs the single-map raycast takes (RunKinds, vm_dev_ray.h) and its
+// register budget: a 640x480 image is 4800 waves, under five per SIMD on 256
+// compute units, so six waves per SIMD keep every wave resident
+constexpr int kRaycastRuns = kRunsInvalid;
+constexpr int kRaycastWavesPerEu = 6;
+
 template <int FMT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_raycast(MapView m, float* vmap, int64_t pitch, int W, int H,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kRaycastWavesPerEu))) void k_raycast(MapView m, float* vmap, int64_t pitch, int W, int H,
                                                 Intrinsics K, Affine pose, Affine Tinv, float vsInv, float rtStep,
                                                 int gate, int tilesX, int tilesPerXcd, int nTiles,
                                                 RayBounds rb, float margin) {
@@ -33,7 +39,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
     float nearLim = 0.0f, farLim = INFINITY;
     if (rb.keys) ray_bounds_limits(rb, t, margin, nearLim, farLim);
     float* out = (float*)((uint8_t*)vmap + (int64_t)y * pitch) + x * 4;
-    raycast_pixel<FMT>(m, out, x, y, K, pose, Tinv, vsInv, rtStep, nearLim, farLim);
+    raycast_pixel<FMT, kRaycastRuns>(m, out, x, y, K, pose, Tinv, vsInv, rtStep, nearLim, farLim);
 }
 
 // Ray-distance bounds of every tile (ray_bounds_chunk), kBoundsChunk
