@@ -225,8 +225,11 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
  * three Newton steps on the trilinear TSDF. Its Jacobian by the twist xi of
  * T exp(xi) is the implicit-surface derivative dt = -grad s . (do + t dd) /
  * (grad s . d) through the ray's origin and direction. Residuals with
- * |r| >= max_residual metres, misses and depth
- * outside [depth_min, depth_max] are left out. Each iteration is one launch
+ * |r| >= max_residual metres, misses and depth outside [depth_min,
+ * min(depth_max, max_depth)] are left out (max_depth <= 0: depth_max) --
+ * with depth noise that grows with range, a max_depth where the noise stays
+ * below half the truncation keeps the fit on the part of the map that
+ * fusion did not bias. Each iteration is one launch
  * over all hypotheses, then a Levenberg-Marquardt decision per hypothesis: a
  * new pose is kept only if it does not raise the truncated cost (sum of r^2
  * below max_residual, max_residual^2 for every other pixel with a valid
@@ -241,8 +244,8 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
  * J^T J row by row, then J^T r. Synchronous. */
 int vm_refine_poses(vm_map* map, const void* depth, int64_t depth_pitch, int32_t width, int32_t height,
                     int32_t depth_format, float depth_scale, const float K[4], const float* init_poses, int32_t n_hyp,
-                    int32_t iters, int32_t stride, float max_residual, float* out_poses, float* out_cost,
-                    double* out_system, void* stream);
+                    int32_t iters, int32_t stride, float max_residual, float max_depth, float* out_poses,
+                    float* out_cost, double* out_system, void* stream);
 
 /* State transfer for tests and tools: copy a whole map array to/from HOST
  * memory. which: 0 hash table, 1 heap, 2 voxels, 3 counters (16 ints),

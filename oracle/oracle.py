@@ -48,6 +48,7 @@ def lib():
         L.or_project_rendering_blocks.argtypes = [_P, _P, _P, C.c_int, C.c_int, _P, _P]
         L.or_raycast_pixels_f32.argtypes = [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P]
         L.or_raycast_pixels_f64.argtypes = [_P, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int, _P, _P, _P]
+        L.or_surface_depth_f64.argtypes = [_P, C.c_int, _P, _P, _P, _P, _P, _P]
         L.or_insert_block.restype = C.c_int
         L.or_insert_block.argtypes = [_P, C.c_int, C.c_int, C.c_int]
         L.or_map_counters.restype = C.c_int
@@ -170,6 +171,20 @@ class OracleMap:
                                     _p(np.ascontiguousarray(T, np.float64).reshape(-1)), _p(pi), _p(pv), len(pi),
                                     _p(v), _p(sig), _p(hit))
         return v, sig, hit.astype(bool)
+
+    def surface_depth_f64(self, px, py, K, T):
+        """(z, ok) per pixel: the pose refinement's depth in FP64 (march,
+        corrections, three Newton steps onto the trilinear zero level);
+        T: 4x4 camera-to-world (row-major) or 16 column-major doubles."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        T = np.asarray(T, np.float64)
+        Tc = np.ascontiguousarray(T.T.reshape(-1) if T.shape == (4, 4) else T.reshape(-1), np.float64)
+        z = np.zeros(len(px), np.float64)
+        ok = np.zeros(len(px), np.int32)
+        lib().or_surface_depth_f64(self.h, len(px), _p(px), _p(py), _p(np.ascontiguousarray(K, np.float64)), _p(Tc),
+                                   _p(z), _p(ok))
+        return z, ok.astype(bool)
 
     def project_rendering_blocks(self, cols8, rows8, K, Twc):
         zx = np.zeros((rows8, cols8), np.float32)

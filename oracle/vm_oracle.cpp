@@ -583,6 +583,25 @@ struct MarchT {
         *valid = vp;
         return ((S)1 - f.z) * r2 + f.z * r3;
     }
+    /* the same trilinear value with its spatial gradient d value / d p */
+    S interp_grad(V3T<S> p, bool* valid, S g[3]) {
+        V3T<S> f{p.x - std::floor(p.x), p.y - std::floor(p.y), p.z - std::floor(p.z)};
+        bool vp = true, v;
+        S c[8];
+        for (int k = 0; k < 8; ++k) {
+            c[k] = read(V3T<S>{p.x + (S)(k & 1), p.y + (S)((k >> 1) & 1), p.z + (S)((k >> 2) & 1)}, &v);
+            vp = vp && v;
+        }
+        const S gx = (S)1 - f.x, gy = (S)1 - f.y, gz = (S)1 - f.z;
+        const S r0 = gx * c[0] + f.x * c[1], r1 = gx * c[2] + f.x * c[3];
+        const S q0 = gx * c[4] + f.x * c[5], q1 = gx * c[6] + f.x * c[7];
+        const S r2 = gy * r0 + f.y * r1, r3 = gy * q0 + f.y * q1;
+        g[0] = gz * (gy * (c[1] - c[0]) + f.y * (c[3] - c[2])) + f.z * (gy * (c[5] - c[4]) + f.y * (c[7] - c[6]));
+        g[1] = gz * (r1 - r0) + f.z * (q1 - q0);
+        g[2] = r3 - r2;
+        *valid = vp;
+        return gz * r2 + f.z * r3;
+    }
 };
 
 template <typename S>
@@ -648,6 +667,68 @@ bool march_pixel(MarchT<S>& mt, int x, int y, const S* K, const S* T, const S* T
     V3T<S> w{res.x / vsInv, res.y / vsInv, res.z / vsInv};
     V3T<S> c = xformT(Ti, w);
     out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = 1;
+    return true;
+}
+
+/* The pose refinement's depth of a pixel (no counterpart in the reference;
+ * it builds on the raycast, ray_tracing.cu:330-399): the march and its first
+ * correction, the second correction's step, then kZeroSteps Newton steps
+ * p <- p - s(p) / (grad s . dir) dir onto the zero level of the trilinear
+ * TSDF; the depth is the camera z of p. Fails where the march misses, an
+ * interpolation is invalid, grad s . dir is within 1e-6 of zero, or the
+ * final |s| is not below 1e-3. T, Ti: column-major camToWorld and inverse. */
+template <typename S>
+bool surface_pixel(MarchT<S>& mt, int x, int y, const S* K, const S* T, const S* Ti, S vsInv, S rtStep, S* z) {
+    const S ifx = (S)(1.0 / (double)K[0]), ify = (S)(1.0 / (double)K[1]), cx = K[2], cy = K[3];
+    const S zNear = (S)0.1f, zFar = (S)5.f;
+    V3T<S> pt{ifx * ((S)x - cx) * zNear, ify * ((S)y - cy) * zNear, zNear};
+    S dist_s = std::sqrt(pt.x * pt.x + (pt.y * pt.y + pt.z * pt.z)) * vsInv;
+    V3T<S> bs = xformT(T, pt);
+    bs = V3T<S>{bs.x * vsInv, bs.y * vsInv, bs.z * vsInv};
+    pt = V3T<S>{ifx * ((S)x - cx) * zFar, ify * ((S)y - cy) * zFar, zFar};
+    const S distEnd = std::sqrt(pt.x * pt.x + (pt.y * pt.y + pt.z * pt.z)) * vsInv;
+    V3T<S> be = xformT(T, pt);
+    be = V3T<S>{be.x * vsInv, be.y * vsInv, be.z * vsInv};
+    V3T<S> dir{be.x - bs.x, be.y - bs.y, be.z - bs.z};
+    const S n2 = dir.x * dir.x + (dir.y * dir.y + dir.z * dir.z);
+    if (n2 > 0) {
+        const S n = std::sqrt(n2);
+        dir = V3T<S>{dir.x / n, dir.y / n, dir.z / n};
+    }
+    V3T<S> res = bs;
+    S sdf = 1, last, step;
+    bool valid = false;
+    while (dist_s < distEnd) {
+        last = sdf;
+        sdf = mt.read(res, &valid);
+        if (sdf <= (S)0.5 && sdf >= (S)-0.5) sdf = mt.interp(res, &valid);
+        if (sdf <= 0) break;
+        if (sdf >= 0 && last < 0) return false;
+        step = valid ? std::max(sdf * rtStep, (S)1) : (S)1;
+        res = V3T<S>{res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z};
+        dist_s += step;
+    }
+    if (!(sdf <= 0)) return false;
+    step = sdf * rtStep;
+    res = V3T<S>{res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z};
+    S g[3];
+    S v = mt.interp_grad(res, &valid, g);
+    if (!valid) return false;
+    step = v * rtStep;
+    res = V3T<S>{res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z};
+    for (int k = 0; k <= 3; ++k) {  /* three Newton steps, then the final check */
+        v = mt.interp_grad(res, &valid, g);
+        const S den = g[0] * dir.x + g[1] * dir.y + g[2] * dir.z;
+        if (!valid || !(std::fabs(den) > (S)1e-6)) return false;
+        if (k == 3) {
+            if (!(std::fabs(v) < (S)1e-3)) return false;
+            break;
+        }
+        const S t = -v / den;
+        res = V3T<S>{res.x + dir.x * t, res.y + dir.y * t, res.z + dir.z * t};
+    }
+    V3T<S> w{res.x / vsInv, res.y / vsInv, res.z / vsInv};
+    *z = xformT(Ti, w).z;
     return true;
 }
 
@@ -968,6 +1049,23 @@ void or_raycast_pixels_f64(or_map* m, int n, const int* px, const int* py, const
         o[0] = NAN; o[1] = o[2] = o[3] = 0;
         hit[i] = march_pixel<double>(mt, px[i], py[i], K, T, Ti, vsInv, rtStep, o);
         sig[i] = mt.sig;
+    }
+}
+
+void or_surface_depth_f64(or_map* m, int n, const int* px, const int* py, const double* K, const double* T,
+                          double* out_z, int* ok) {
+    double Ti[16] = {0};
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) Ti[c * 4 + r] = T[r * 4 + c];
+        Ti[12 + r] = -(T[r * 4 + 0] * T[12] + T[r * 4 + 1] * T[13] + T[r * 4 + 2] * T[14]);
+    }
+    Ti[15] = 1;
+    const double vsInv = 1.0 / (double)m->voxelSize, rtStep = (double)m->trunc / (double)m->voxelSize;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(g_threads)
+    for (int i = 0; i < n; ++i) {
+        MarchT<double> mt{m, Perturb{nullptr, nullptr, 0}, 0ull};
+        out_z[i] = NAN;
+        ok[i] = surface_pixel<double>(mt, px[i], py[i], K, T, Ti, vsInv, rtStep, out_z + i);
     }
 }
 

@@ -76,6 +76,10 @@ void or_raycast_pixels_f32(or_map* m, int n, const int* px, const int* py, const
 void or_raycast_pixels_f64(or_map* m, int n, const int* px, const int* py, const double* K, const double* T,
                            const int64_t* pidx, const double* pval, int npert, double* out_v, uint64_t* sig,
                            int* hit);
+/* the pose refinement's depth (march, corrections, three Newton steps onto
+ * the trilinear zero level) of pixels in FP64; T column-major double */
+void or_surface_depth_f64(or_map* m, int n, const int* px, const int* py, const double* K, const double* T,
+                          double* out_z, int* ok);
 
 /* createBlock (map_struct.h:304-354) for one key; returns its voxel ptr or -1 */
 int or_insert_block(or_map* m, int x, int y, int z);

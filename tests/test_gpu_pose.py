@@ -1,7 +1,11 @@
 """Batched Gauss-Newton pose refinement (vm_refine_poses).
 
 * The gradient J^T r of the normal equations equals central finite
-  differences of the cost over the six twist components.
+  differences of the cost over the six twist components: of the kernel's
+  own FP32 cost, and of the oracle's FP64 restatement of the depth
+  (or_surface_depth_f64: march, corrections, Newton steps onto the zero
+  level) on a noisy room_vga map, with the residual set and the cost
+  compared too.
 * Hypotheses perturbed by up to 2 cm / 1.5 degrees around the true pose of a
   well-conditioned view converge to it, with the cost decreasing.
 * Arguments are validated."""
@@ -84,6 +88,52 @@ def test_gradient_matches_finite_differences():
     rel = np.abs(fd - grad) / np.abs(grad).max()
     print(f"gradient {grad}\nfinite differences {fd}\nrel {rel}")
     assert rel.max() < 5e-2
+
+
+def test_gradient_matches_fp64_host_evaluation(oracle_threads):
+    """room_vga (640x480 uint16 depth with its noise and dropout), 12 frames
+    fused into the GPU map and the oracle; frame 12's depth against a pose
+    1 cm / 0.5 degrees off, every 8th pixel: the residual count and the cost
+    match the FP64 host evaluation, and J^T r matches FP64 central
+    differences of 0.5 sum r^2 over the six twist components (h = 1e-6)."""
+    from voxelmapping_amd import synth
+    from voxelmapping_amd.autograd import se3_exp
+    from parity import gpu_frame
+    w = synth.room_vga()
+    g, o = make_pair(w)
+    fuse_both(w, g, o, range(12), rgb=False)
+    d, _, dt, _ = gpu_frame(w, 12, rgb=False)
+    rng = np.random.default_rng(5)
+    T32 = perturb(true_pose(w, 12), rng, 0.01, 0.5).astype(np.float32)
+    _, cost, system = g.refine_poses(dt, w.K, T32[None], iters=0, stride=8, max_residual=0.05,
+                                     depth_scale=w.depth_scale, return_system=True)
+    n_gpu, mse_gpu, jtr_gpu = int(cost[0, 0, 1]), float(cost[0, 0, 0]), system[0, 0, 21:27]
+    ys, xs = np.mgrid[0:w.height:8, 0:w.width:8]
+    dep = w.depth_metres(d)[ys, xs].astype(np.float64)
+    keep = (dep >= 0.1) & (dep <= 5.0)
+    xs, ys, dep = xs[keep], ys[keep], dep[keep]
+    K = np.asarray(w.K, np.float64)
+    T = T32.astype(np.float64)
+    z, ok = o.surface_depth_f64(xs, ys, K, T)
+    r = z - dep
+    inset = ok & (np.abs(r) < 0.05)
+    n_host = int(inset.sum())
+    print(f"residuals gpu {n_gpu} host {n_host}; mean r^2 gpu {mse_gpu:.6e} host {np.mean(r[inset] ** 2):.6e}")
+    assert n_gpu > 2000 and abs(n_gpu - n_host) <= 0.002 * n_host + 2
+    assert abs(mse_gpu - np.mean(r[inset] ** 2)) <= 0.01 * mse_gpu
+    h = 1e-6
+    fd = np.zeros(6)
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        zp, okp = o.surface_depth_f64(xs, ys, K, T @ se3_exp(e))
+        zm, okm = o.surface_depth_f64(xs, ys, K, T @ se3_exp(-e))
+        both = inset & okp & okm
+        assert both.sum() >= inset.sum() - 2
+        fd[k] = np.sum(r[both] * (zp[both] - zm[both]) / (2 * h))
+    rel = np.abs(fd - jtr_gpu) / np.abs(fd).max()
+    print(f"J^T r gpu {jtr_gpu}\nfp64 central differences {fd}\nrelative error {rel}")
+    assert rel.max() < 1e-2
 
 
 def test_hypotheses_converge():

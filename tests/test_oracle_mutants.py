@@ -80,6 +80,9 @@ MUTANTS = {
                       Lx * Ks * Sx * std::pow(std::fmax(0.f, dot3(R, Vv)), n);""",
         """            float I = Ax * Ka * Dx + Lx * Kd * Dx * std::fmax(0.f, nl);""",
         ["test_render_shading_closed_form"]),
+    "refinement_newton_steps_skipped": (
+        "        const S t = -v / den;", "        const S t = 0 * v / den;",
+        ["test_surface_depth_f64_sphere"]),
     "rendering_block_floor_for_ceil": (
         "if ((float)lrx < std::ceil(px)) lrx = (short)f2i_sat(std::ceil(px));",
         "if ((float)lrx < std::floor(px)) lrx = (short)f2i_sat(std::floor(px));",

@@ -142,7 +142,7 @@ SIGNATURES = {
     "vm_map_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "vm_map_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "vm_refine_poses": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_float, _F4,
-                                  C.POINTER(C.c_float), C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                  C.POINTER(C.c_float), C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_double), _P]),
     "vm_map_download_blocks": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
     "vm_map_upload_blocks": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),

@@ -656,7 +656,8 @@ int vm_raycast_backward(vm_map* m, const float* vmap, int64_t vpitch, const floa
 
 int vm_refine_poses(vm_map* m, const void* depth, int64_t pitch, int32_t W, int32_t H, int32_t fmt, float scale,
                     const float K[4], const float* init_poses, int32_t n_hyp, int32_t iters, int32_t stride,
-                    float max_residual, float* out_poses, float* out_cost, double* out_system, void* stream) {
+                    float max_residual, float max_depth, float* out_poses, float* out_cost, double* out_system,
+                    void* stream) {
     if (check_map(m)) return VM_ERR_INVALID_ARG;
     if (fmt != VM_DEPTH_F32 && fmt != VM_DEPTH_U16) return fail(VM_ERR_INVALID_ARG, "unknown depth_format");
     if (int rc = check_image(depth, pitch, W, H, fmt == VM_DEPTH_F32 ? 4 : 2, "depth")) return rc;
@@ -664,6 +665,7 @@ int vm_refine_poses(vm_map* m, const void* depth, int64_t pitch, int32_t W, int3
     if (iters < 0 || iters > 100) return fail(VM_ERR_INVALID_ARG, "iters must be in [0, 100]");
     if (stride < 1) return fail(VM_ERR_INVALID_ARG, "stride must be >= 1");
     if (!(max_residual > 0.f)) return fail(VM_ERR_INVALID_ARG, "max_residual must be > 0");
+    if (!(max_depth == max_depth)) return fail(VM_ERR_INVALID_ARG, "max_depth must not be NaN");
     if (!init_poses || !out_poses || !out_cost)
         return fail(VM_ERR_INVALID_ARG, "init_poses, out_poses and out_cost must not be NULL");
     // column-major float -> row-major double; every hypothesis must be finite
@@ -687,7 +689,10 @@ int vm_refine_poses(vm_map* m, const void* depth, int64_t pitch, int32_t W, int3
     double* posesD = nullptr;
     float* cost = nullptr;
     double* system = nullptr;
-    VM_HIP(launch_refine_poses(m->view, d, intrinsics_of(K), Tin.data(), n_hyp, iters, stride, max_residual, 1e-3,
+    // pixels deeper than max_depth are left out like those beyond depth_max
+    MapView view = m->view;
+    if (max_depth > 0.f && max_depth < view.depthMax) view.depthMax = max_depth;
+    VM_HIP(launch_refine_poses(view, d, intrinsics_of(K), Tin.data(), n_hyp, iters, stride, max_residual, 1e-3,
                                m->scratch, &posesD, &cost, &system, s),
            "pose refinement");
     if (out_system)

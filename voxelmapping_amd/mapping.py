@@ -250,10 +250,11 @@ class DeviceMap:
                 4: 20 * d.n_entry, 5: 16 * d.n_block}[which]
 
     def refine_poses(self, depth, K, init_poses, iters: int = 8, stride: int = 4, max_residual: float = 0.05,
-                     depth_scale: float = 1e-3, stream=None, return_system: bool = False):
+                     depth_scale: float = 1e-3, stream=None, return_system: bool = False, max_depth: float = 0.0):
         """Gauss-Newton refinement of camera-to-world hypotheses against one
         depth image (vm_refine_poses). init_poses: (n, 4, 4) matrices or
-        (n, 16) column-major. Returns (poses (n, 4, 4) float32, cost
+        (n, 16) column-major. Pixels deeper than max_depth metres (if > 0)
+        are left out. Returns (poses (n, 4, 4) float32, cost
         (n, iters + 1, 2) = {mean squared residual, count} per evaluation),
         plus the normal equations (n, iters + 1, 27) if return_system."""
         p, pitch, W, H, fmt, sc = self._depth(depth, depth_scale)
@@ -269,7 +270,8 @@ class DeviceMap:
         system = np.zeros((n, iters + 1, 27), np.float64)
         fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
         check(lib().vm_refine_poses(self._h, p, pitch, W, H, fmt, sc, intrinsics(K), fp(P), n, iters, stride,
-                                    max_residual, fp(out), fp(cost), system.ctypes.data_as(C.POINTER(C.c_double)),
+                                    max_residual, max_depth, fp(out), fp(cost),
+                                    system.ctypes.data_as(C.POINTER(C.c_double)),
                                     _stream(stream)))
         poses = out.reshape(n, 4, 4).transpose(0, 2, 1).copy()
         return (poses, cost, system) if return_system else (poses, cost)
