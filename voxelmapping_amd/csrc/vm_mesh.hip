@@ -58,6 +58,21 @@ __global__ void k_mc_links(MapView m, int* links, int capBlocks) {
     }
 }
 
+// Blocks of workgroup w of a g-workgroup counting or emitting pass (g a
+// multiple of 8). The dispatcher places workgroup w on XCD w % 8; XCD x
+// takes the contiguous block-id range [A x / 8, A (x + 1) / 8) -- blocks
+// allocated together, mostly neighbours in space -- and its workgroups step
+// through it side by side, so the face layers a block reads from its
+// neighbours were mostly fetched into that XCD's own L2 moments before.
+struct McRange {
+    int first, end, step;
+};
+__device__ inline McRange mc_range(int w, int g, int A) {
+    const int x = w & 7;
+    const int lo = (int)((int64_t)A * x / 8), hi = (int)((int64_t)A * (x + 1) / 8);
+    return McRange{lo + (w >> 3), hi, g >> 3};
+}
+
 // LDS of the counting and emitting passes
 struct McShared {
     float sdf[kHalo];
@@ -160,14 +175,15 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* l
     const int tid = threadIdx.x;
     mc_load_tables(sh, tid);
     const int A = mesh_blocks(m.ctr, m.nBlock, capBlocks);
+    const McRange r = mc_range(blockIdx.x, gridDim.x, A);
     // per-lane (vector) loads: their wait is at the use, one block later
     int nbNext = -1;
-    if ((int)blockIdx.x < A && tid < 8) nbNext = links[(int64_t)blockIdx.x * 8 + tid];
+    if (r.first < r.end && tid < 8) nbNext = links[(int64_t)r.first * 8 + tid];
     int ph = 0;
-    for (int b = blockIdx.x; b < A; b += gridDim.x, ph ^= 1) {
+    for (int b = r.first; b < r.end; b += r.step, ph ^= 1) {
         const int nb = nbNext;
-        const int bn = b + gridDim.x;
-        if (bn < A && tid < 8) nbNext = links[(int64_t)bn * 8 + tid];
+        const int bn = b + r.step;
+        if (bn < r.end && tid < 8) nbNext = links[(int64_t)bn * 8 + tid];
         if (tid < 8) sh.nb[ph][tid] = nb;
         __syncthreads();
         if (sh.nb[ph][0] == -2) {  // dead block
@@ -182,8 +198,9 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_count(MapView m, const int* l
 }
 
 // Offsets of the emitting pass's g workgroups (one 1024-thread workgroup):
-// workgroup w takes blocks w, w + g, ..., so its count is the sum of theirs;
-// an exclusive scan over w gives the offsets, the sum the triangle count.
+// workgroup w takes the blocks of mc_range(w, g, A), so its count is the sum
+// of theirs; an exclusive scan over w gives the offsets, the sum the
+// triangle count.
 // Offsets and the total are 64-bit (2^22 blocks of up to 2560 triangles
 // exceed 2^32); a workgroup's own sum, at most 2560 per block of its share,
 // fits 32 bits.
@@ -198,8 +215,10 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, un
     for (int c0 = 0; c0 < g; c0 += 1024) {
         const int i = c0 + tid;
         unsigned v = 0u;
-        if (i < g)
-            for (int b = i; b < A; b += g) v += (unsigned)blockTri[b];
+        if (i < g) {
+            const McRange r = mc_range(i, g, A);
+            for (int b = r.first; b < r.end; b += r.step) v += (unsigned)blockTri[b];
+        }
         unsigned long long incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -229,7 +248,7 @@ __global__ __launch_bounds__(1024) void k_mc_scan(const int* blockTri, int g, un
 }
 
 // Emitting pass, one 256-thread workgroup per block, workgroup w taking
-// blocks w, w + grid, ... (the order k_mc_scan assumes): blocks without
+// the blocks of mc_range(w, grid, A) (the order k_mc_scan assumes): blocks without
 // triangles are skipped; for the others
 // the halo is read and classified again, an LDS map lists (cube, k) per
 // triangle, each thread builds one triangle per pass of 256, staged in LDS,
@@ -249,6 +268,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_emit(MapView m, const int* li
     const int A = mesh_blocks(m.ctr, m.nBlock, capBlocks);
     const float vs = m.voxelSize;
     int64_t base = (int64_t)wgBase[blockIdx.x];
+    const McRange r = mc_range(blockIdx.x, gridDim.x, A);
     // per-lane (vector) loads: their wait is at the use, one block later
     int nbNext = -1, kcNext = 0;
     auto prefetch = [&](int bb) {
@@ -256,11 +276,11 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_emit(MapView m, const int* li
         if (tid < 3) kcNext = ((const int*)m.blockPos)[(int64_t)bb * 4 + tid];
         if (tid == 3) kcNext = blockTri[bb];
     };
-    if ((int)blockIdx.x < A) prefetch(blockIdx.x);
+    if (r.first < r.end) prefetch(r.first);
     int ph = 0;
-    for (int b = blockIdx.x; b < A; b += gridDim.x, ph ^= 1) {
+    for (int b = r.first; b < r.end; b += r.step, ph ^= 1) {
         const int nb = nbNext, kc = kcNext;
-        if (b + (int)gridDim.x < A) prefetch(b + gridDim.x);
+        if (b + r.step < r.end) prefetch(b + r.step);
         if (tid < 8) sh.nb[ph][tid] = nb;
         if (tid < 4) sh.key[ph][tid] = kc;
         __syncthreads();
@@ -366,9 +386,12 @@ __global__ void k_shading(const float* vmap, int64_t vpitch, const float* nmap, 
 
 }  // namespace
 
+// workgroups of a counting or emitting pass: perCu per compute unit, fewer
+// for a small pool, a multiple of 8 (mc_range)
 static int mc_groups(int nBlock, int perCu) {
-    const int g = num_compute_units() * perCu;
-    return g < nBlock ? g : (nBlock > 0 ? nBlock : 1);
+    int g = num_compute_units() * perCu;
+    if (g > nBlock) g = nBlock;
+    return ((g + 7) / 8) * 8;
 }
 
 int polygonize_groups(int nBlock) { return mc_groups(nBlock, kMcGroupsPerCu); }
