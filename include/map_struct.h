@@ -34,7 +34,10 @@ struct VoxelRGB {
     unsigned char wt;
     Vector3b rgb;
 };
-// float TSDF / float weight / colour (no counterpart in the reference)
+// float TSDF / float weight / colour (no counterpart in the reference): the
+// record vm_map_download / vm_map_upload transfer. In device memory a
+// MapStruct<VoxelF32>'s voxelBlock holds each block as 512 {sdf, wt} float
+// pairs followed by 512 {rgb, pad} words (VM_VOXEL_F32_F32_RGB8).
 struct VoxelF32 {
     float sdf;
     float wt;

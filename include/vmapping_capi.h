@@ -41,7 +41,11 @@ extern "C" {
 /* voxel storage formats */
 #define VM_VOXEL_I16_U8 0     /* vmap::Voxel    {short sdf; uchar wt}            4 B */
 #define VM_VOXEL_I16_U8_RGB 1 /* vmap::VoxelRGB {short sdf; uchar wt; uchar[3]}  6 B */
-#define VM_VOXEL_F32_F32_RGB8 2 /* {float sdf; float wt; uchar rgb[3]; pad}     12 B */
+#define VM_VOXEL_F32_F32_RGB8 2 /* {float sdf; float wt; uchar rgb[3]; pad}     12 B;
+                                   in device memory each block holds its 512
+                                   {sdf, wt} pairs (4 KiB), then its 512
+                                   {rgb, pad} words (2 KiB); transfers to and
+                                   from the host use the 12-byte records */
 
 /* depth image formats */
 #define VM_DEPTH_F32 0 /* float metres (the reference's GpuMat CV_32FC1) */
@@ -81,7 +85,7 @@ typedef struct vm_map_arrays {
     int32_t* excess_ptr;  /* int */
     int32_t* bucket_mutex;/* int[n_bucket] */
     uint32_t* visible_block_num; /* uint */
-    void* voxel_block;    /* voxels[n_block * 512] */
+    void* voxel_block;    /* voxels[n_block * 512] (the float format's blocks split, see above) */
     int32_t* block_pos;   /* int4[n_block] {x, y, z, slot} of allocated blocks */
 } vm_map_arrays;
 
@@ -249,8 +253,10 @@ int vm_refine_poses(vm_map* map, const void* depth, int64_t depth_pitch, int32_t
 
 /* State transfer for tests and tools: copy a whole map array to/from HOST
  * memory. which: 0 hash table, 1 heap, 2 voxels, 3 counters (16 ints),
- * 4 visible table, 5 block positions. vm_map_upload rebuilds the block
- * index from the hash table after uploading the hash table. Synchronous. */
+ * 4 visible table, 5 block positions. Voxels travel as the format's records
+ * (12-byte {sdf, wt, rgb, pad} for VM_VOXEL_F32_F32_RGB8). vm_map_upload
+ * rebuilds the block index from the hash table after uploading the hash
+ * table. Synchronous. */
 int vm_map_download(vm_map* map, int32_t which, void* host_dst, int64_t bytes);
 int vm_map_upload(vm_map* map, int32_t which, const void* host_src, int64_t bytes);
 /* Voxels of blocks [first_block, first_block + n_blocks) to/from HOST memory

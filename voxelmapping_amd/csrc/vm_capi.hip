@@ -718,6 +718,55 @@ int vm_surface_points(const float* verts, float* points, int64_t n, void* stream
     return VM_OK;
 }
 
+// Voxels of blocks [first, first + n) between the device pool and HOST
+// memory. The float format travels as 12-byte records {float sdf; float wt;
+// uchar rgb[3]; uchar pad}; on the device its blocks hold the {sdf, wt}
+// pairs and the colour words as two arrays (Vox<kFmtF32>), so it is
+// converted in chunks through a host buffer. The other formats are copied
+// as they are.
+static int voxels_transfer(vm_map* m, int64_t first, int64_t n, void* host, bool toHost) {
+    const MapView& v = m->view;
+    const size_t per = (size_t)voxel_bytes(v.fmt) * kBlockVox;
+    uint8_t* dev = v.voxels + (size_t)first * per;
+    VM_HIP(hipDeviceSynchronize(), "synchronize");
+    if (n <= 0) return VM_OK;
+    if (v.fmt != kFmtF32) {
+        VM_HIP(hipMemcpy(toHost ? host : (void*)dev, toHost ? (const void*)dev : host, (size_t)n * per,
+                         toHost ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice),
+               toHost ? "download" : "upload");
+        return VM_OK;
+    }
+    const int64_t chunk = 8192;  // blocks per pass: 48 MiB
+    std::vector<uint8_t> tmp((size_t)std::min<int64_t>(n, chunk) * per);
+    const size_t colour = (size_t)Vox<kFmtF32>::kColorOffset;
+    for (int64_t b0 = 0; b0 < n; b0 += chunk) {
+        const int64_t c = std::min<int64_t>(chunk, n - b0);
+        uint8_t* h = (uint8_t*)host + (size_t)b0 * per;
+        if (toHost) {
+            VM_HIP(hipMemcpy(tmp.data(), dev + (size_t)b0 * per, (size_t)c * per, hipMemcpyDeviceToHost), "download");
+            for (int64_t k = 0; k < c; ++k) {
+                const uint8_t* blk = tmp.data() + (size_t)k * per;
+                uint8_t* rec = h + (size_t)k * per;
+                for (int l = 0; l < kBlockVox; ++l) {
+                    std::memcpy(rec + (size_t)l * 12, blk + (size_t)l * 8, 8);
+                    std::memcpy(rec + (size_t)l * 12 + 8, blk + colour + (size_t)l * 4, 4);
+                }
+            }
+        } else {
+            for (int64_t k = 0; k < c; ++k) {
+                uint8_t* blk = tmp.data() + (size_t)k * per;
+                const uint8_t* rec = h + (size_t)k * per;
+                for (int l = 0; l < kBlockVox; ++l) {
+                    std::memcpy(blk + (size_t)l * 8, rec + (size_t)l * 12, 8);
+                    std::memcpy(blk + colour + (size_t)l * 4, rec + (size_t)l * 12 + 8, 4);
+                }
+            }
+            VM_HIP(hipMemcpy(dev + (size_t)b0 * per, tmp.data(), (size_t)c * per, hipMemcpyHostToDevice), "upload");
+        }
+    }
+    return VM_OK;
+}
+
 static int array_of(vm_map* m, int which, void** p, int64_t* bytes) {
     const MapView& v = m->view;
     switch (which) {
@@ -737,6 +786,7 @@ int vm_map_download(vm_map* m, int32_t which, void* dst, int64_t bytes) {
     int64_t n;
     if (int rc = array_of(m, which, &p, &n)) return rc;
     if (!dst || bytes != n) return fail(VM_ERR_INVALID_ARG, "host buffer size does not match the array");
+    if (which == 2) return voxels_transfer(m, 0, m->view.nBlock, dst, true);
     VM_HIP(hipDeviceSynchronize(), "synchronize");
     VM_HIP(hipMemcpy(dst, p, (size_t)n, hipMemcpyDeviceToHost), "download");
     return VM_OK;
@@ -773,8 +823,12 @@ int vm_map_upload(vm_map* m, int32_t which, const void* src, int64_t bytes) {
         for (int i = 0; i < v.nBlock; ++i)
             if (b[i].w < -1 || b[i].w >= v.nEntry) return fail(VM_ERR_INVALID_ARG, "block position out of range");
     }
-    VM_HIP(hipDeviceSynchronize(), "synchronize");
-    VM_HIP(hipMemcpy(p, src, (size_t)n, hipMemcpyHostToDevice), "upload");
+    if (which == 2) {
+        if (int rc = voxels_transfer(m, 0, v.nBlock, const_cast<void*>(src), false)) return rc;
+    } else {
+        VM_HIP(hipDeviceSynchronize(), "synchronize");
+        VM_HIP(hipMemcpy(p, src, (size_t)n, hipMemcpyHostToDevice), "upload");
+    }
     if (which == 0) {
         VM_HIP(launch_rebuild_index(m->view, nullptr), "block index rebuild");
         VM_HIP(hipStreamSynchronize(nullptr), "synchronize");
@@ -799,17 +853,14 @@ int vm_map_download_blocks(vm_map* m, int32_t first_block, int32_t n_blocks, voi
     uint8_t* p;
     size_t n;
     if (int rc = voxel_range(m, first_block, n_blocks, dst, &p, &n)) return rc;
-    VM_HIP(hipDeviceSynchronize(), "synchronize");
-    if (n) VM_HIP(hipMemcpy(dst, p, n, hipMemcpyDeviceToHost), "download");
-    return VM_OK;
+    return voxels_transfer(m, first_block, n_blocks, dst, true);
 }
 
 int vm_map_upload_blocks(vm_map* m, int32_t first_block, int32_t n_blocks, const void* src) {
     uint8_t* p;
     size_t n;
     if (int rc = voxel_range(m, first_block, n_blocks, src, &p, &n)) return rc;
-    VM_HIP(hipDeviceSynchronize(), "synchronize");
-    if (n) VM_HIP(hipMemcpy(p, src, n, hipMemcpyHostToDevice), "upload");
+    if (int rc = voxels_transfer(m, first_block, n_blocks, const_cast<void*>(src), false)) return rc;
     m->epoch++;
     m->visValid = false;
     return VM_OK;

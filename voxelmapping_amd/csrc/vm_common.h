@@ -54,7 +54,7 @@ enum StatusBits : int {
 enum VoxelFormat : int {
     kFmtRef = 0,     // vmap::Voxel    {short sdf; uchar wt}           4 B
     kFmtRefRGB = 1,  // vmap::VoxelRGB {short sdf; uchar wt; uchar3}   6 B
-    kFmtF32 = 2,     // {float sdf; float wt; uchar rgb[3]; uchar pad} 12 B
+    kFmtF32 = 2,     // float sdf, float wt, uchar rgb[3] + pad: 12 B, split per block (Vox<kFmtF32>)
 };
 
 // Plain-old-data view of one map, passed to kernels by value.
@@ -218,14 +218,21 @@ template <> struct Vox<kFmtRefRGB> {
         valid = (p[1] & 0xffu) != 0;
     }
 };
-// float TSDF + float weight + rgb8
+// float TSDF + float weight + rgb8, 12 B per voxel, stored per block as two
+// arrays: 512 {sdf, weight} float pairs (4 KiB), then 512 colour words
+// {r, g, b, pad} (2 KiB). Voxel i (a block's ptr + local index) is local
+// i % 512 of block i / 512. Readers of the field (raycast, adjoint, pose,
+// marching cubes) fetch one 8-byte pair and no colour bytes.
 template <> struct Vox<kFmtF32> {
     static constexpr int kBytes = 12;
     static constexpr bool kHasColor = true;
+    static constexpr int64_t kBlockBytes = (int64_t)kBlockVox * 12;
+    static constexpr int kColorOffset = kBlockVox * 8;  // of the colour words within a block
+    __host__ __device__ static inline int64_t field_offset(int64_t i) { return (i >> 9) * kBlockBytes + (i & 511) * 8; }
     __device__ static inline void read(const uint8_t* base, int64_t i, float& sdf, bool& valid) {
-        const float* p = (const float*)(base + i * 12);
-        sdf = p[0];
-        valid = p[1] != 0.f;
+        const float2 v = *(const float2*)(base + field_offset(i));
+        sdf = v.x;
+        valid = v.y != 0.f;
     }
 };
 

@@ -295,34 +295,35 @@ __device__ inline void integrate_block(const MapView& m, const DepthImage& d, co
     uint8_t* base = m.voxels + ((int64_t)blk * kBlockVox + lane) * Vox<FMT>::kBytes;
     constexpr int64_t kSlice = 64 * Vox<FMT>::kBytes;
     if constexpr (FMT == kFmtF32) {
-        float os[8], ow[8];
+        // the block's {sdf, weight} pairs and colour words, z-slice z at + 64 z
+        uint8_t* blockBase = m.voxels + (int64_t)blk * Vox<kFmtF32>::kBlockBytes;
+        float2* fld = (float2*)blockBase + lane;
+        uint32_t* col = (uint32_t*)(blockBase + Vox<kFmtF32>::kColorOffset) + lane;
+        float2 ov[8];
         uint32_t oc[8];
 #pragma unroll
         for (int z = 0; z < 8; ++z) {
-            const float* f = (const float*)(base + z * kSlice);
-            os[z] = f[0];
-            ow[z] = f[1];
-            oc[z] = COLOR ? ((const uint32_t*)f)[2] : 0u;
+            ov[z] = fld[z * 64];
+            oc[z] = COLOR ? col[z * 64] : 0u;
         }
 #pragma unroll
         for (int z = 0; z < 8; ++z) {
             if (!((ok >> z) & 1u)) continue;
-            float* f = (float*)(base + z * kSlice);
             const float sdf = sdfNew[z];
+            const float os = ov[z].x, ow = ov[z].y;
             float ns, nw;
-            if (ow[z] == 0.f) { ns = sdf; nw = 1.f; }
-            else { ns = (os[z] * ow[z] + sdf * 1.0f) / (ow[z] + 1.0f); nw = fminf(m.maxWeight, ow[z] + 1.0f); }
-            f[0] = ns;
-            f[1] = nw;
+            if (ow == 0.f) { ns = sdf; nw = 1.f; }
+            else { ns = (os * ow + sdf * 1.0f) / (ow + 1.0f); nw = fminf(m.maxWeight, ow + 1.0f); }
+            fld[z * 64] = make_float2(ns, nw);
             if constexpr (COLOR) {
                 const uint32_t old = oc[z];
                 uint32_t r = rgb[z] & 0xffu, g = (rgb[z] >> 8) & 0xffu, b = rgb[z] >> 16;
-                if (ow[z] != 0.f) {
+                if (ow != 0.f) {
                     r = (uint32_t)(0.7f * (float)(old & 0xffu) + 0.3f * (float)r);
                     g = (uint32_t)(0.7f * (float)((old >> 8) & 0xffu) + 0.3f * (float)g);
                     b = (uint32_t)(0.7f * (float)((old >> 16) & 0xffu) + 0.3f * (float)b);
                 }
-                ((uint32_t*)f)[2] = (old & 0xff000000u) | r | (g << 8) | (b << 16);
+                col[z * 64] = (old & 0xff000000u) | r | (g << 8) | (b << 16);
             }
         }
     } else if constexpr (FMT == kFmtRef) {
