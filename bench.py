@@ -154,16 +154,20 @@ def batched_bench(warmup: int, steps: int) -> dict:
         stream.synchronize()
         t_plain = time.perf_counter() - t0
         v_plain = sums() - v0
-        # batched: same maps from scratch, autotune inside the warmup frames
+        # autotune over the first frames (18 launch shapes, 3 scene-group
+        # counts and the one-kernel step, 3 steps each), then the batched
+        # steps over the same frames as the plain loop, from empty maps
+        tune_steps = 3
+        if 22 * tune_steps > nfr:
+            raise SystemExit("the autotuner needs 66 frames")
         for m in maps:
             m.reset(stream)
         b.set_frame(0, stream)
-        tune_steps = 1
-        best = b.tune(steps=tune_steps, stream=stream)  # 18 launch shapes + 3 scene-group counts, 1 step each
-        done = 21 * tune_steps
-        if done > warmup:
-            raise SystemExit("warmup must cover the autotuner's steps")
-        for _ in range(warmup - done):
+        best = b.tune(steps=tune_steps, stream=stream)
+        for m in maps:
+            m.reset(stream)
+        b.set_frame(0, stream)
+        for _ in range(warmup):
             b.step(stream)
         b.capture(stream)
         stream.synchronize()
@@ -174,6 +178,25 @@ def batched_bench(warmup: int, steps: int) -> dict:
         stream.synchronize()
         t_batch = time.perf_counter() - t0
         v_batch = sums() - v0
+        # the same frames again with the step as one kernel (grid barriers
+        # between the stages), graph-replayed: reported beside the tuned step
+        for m in maps:
+            m.reset(stream)
+        b.set_frame(0, stream)
+        b.set_tuning(**{**best, "scene_groups": 1, "persistent": 0})
+        for _ in range(warmup):
+            b.step(stream)
+        b.set_tuning(**{**best, "scene_groups": 1, "persistent": 1, "persist_wg_per_cu": 4})
+        b.capture(stream)
+        stream.synchronize()
+        v0 = sums()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.step(stream)
+        stream.synchronize()
+        t_one = time.perf_counter() - t0
+        v_one = sums() - v0
+        b.frame(stream)  # raises if a grid-barrier wait ran out
     n = len(ws)
     P_total = sum(w.width * w.height for w in ws)
     bytes_frame_pixels = P_total * (B_DEPTH + B_VMAP + B_NMAP)  # no colour in this workload
@@ -191,6 +214,10 @@ def batched_bench(warmup: int, steps: int) -> dict:
         "hbm_frac": round(batch_bytes / t_batch / 1e9 / HBM_PEAK_GBS, 4),
         "visible_blocks_equal": bool(v_plain == v_batch),
         "tuning": best, "graph": True,
+        "one_kernel_step": {"ms_per_step": round(1000 * t_one / steps, 4),
+                            "over_tuned_step": round(t_one / t_batch, 3),
+                            "visible_blocks_equal": bool(v_one == v_batch),
+                            "persist_wg_per_cu": 4},
     }
 
 

@@ -309,12 +309,19 @@ typedef struct vm_batch_tuning {
     int32_t scene_groups;         /* 1..4: scenes dealt to this many groups (largest image first, to the
                                      group with the fewest pixels), each group's kernels on a stream of
                                      its own, forked from and joined back into the caller's stream */
+    int32_t persistent;           /* 0: a step is five kernels; 1: one kernel whose workgroups are all
+                                     resident, the stages separated by grid barriers (needs
+                                     scene_groups 1); 2: the same, launched cooperatively (not capturable) */
+    int32_t persist_wg_per_cu;    /* 1..32: resident workgroups per compute unit of the one-kernel step,
+                                     capped by its occupancy */
 } vm_batch_tuning;
 
 int vm_batch_create(const vm_batch_scene* scenes, int32_t n_scenes, vm_batch** out_batch);
 int vm_batch_destroy(vm_batch* batch);
 /* device-side frame index used by the next step (stream-ordered) */
 int vm_batch_set_frame(vm_batch* batch, int32_t frame, void* stream);
+/* reads the frame index back (synchronizes the stream); VM_ERR_STATE when a
+   one-kernel step's grid-barrier wait has run out (the frame is still set) */
 int vm_batch_get_frame(vm_batch* batch, void* stream, int32_t* out_frame);
 /* one step for all scenes, then frame += 1; replays the captured graph if any */
 int vm_batch_step(vm_batch* batch, void* stream);
@@ -325,9 +332,10 @@ int vm_batch_get_tuning(vm_batch* batch, vm_batch_tuning* out_tuning);
 int vm_batch_set_tuning(vm_batch* batch, const vm_batch_tuning* tuning);
 /* Autotuner: times the 18 launch shapes (ray tile 8x8 / 16x4 / 32x2, XCD
  * bands on / off, 4 / 8 / 16 integration workgroups per CU) with one scene
- * group, then 2, 3 and 4 scene groups with the fastest shape, `steps` steps
- * each (the steps fuse frames like any step), and applies the fastest; drops
- * a captured graph. */
+ * group, then 2, 3 and 4 scene groups with the fastest shape, then the
+ * one-kernel step (persistent 1, 4 workgroups per CU) with that shape,
+ * `steps` steps each (the steps fuse frames like any step: up to 22 steps
+ * times `steps`), and applies the fastest; drops a captured graph. */
 int vm_batch_tune(vm_batch* batch, void* stream, int32_t steps, vm_batch_tuning* out_best);
 
 #ifdef __cplusplus

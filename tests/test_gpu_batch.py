@@ -149,6 +149,43 @@ def test_batch_scene_groups_on_streams():
         b.close()
 
 
+def test_batch_one_kernel_step():
+    """The step as one kernel (the five stages separated by grid barriers):
+    plain launches replayed from a graph, and cooperative launches, equal the
+    plain pipeline bit for bit; its tuning is checked."""
+    import torch
+    from voxelmapping_amd import VMError
+    ws = scenes_spec()
+    for mode, graph in ((1, True), (2, False)):
+        b, maps = build_batch(ws, 8)
+        with pytest.raises(VMError):
+            b.set_tuning(persistent=mode, scene_groups=2)
+        with pytest.raises(VMError):
+            b.set_tuning(persistent=mode, persist_wg_per_cu=0)
+        b.set_tuning(persistent=mode, persist_wg_per_cu=4)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            b.step(stream=s)
+            if graph:
+                b.capture(stream=s)
+            else:
+                with pytest.raises(VMError):
+                    b.capture(stream=s)
+            for _ in range(4):
+                b.step(stream=s)
+                s.synchronize()
+                torch.cuda.synchronize()
+        s.synchronize()
+        assert b.frame(stream=s) == 5  # also raises if a barrier wait ran out
+        for i, w in enumerate(ws):
+            m, vm, nm = plain_run(w, range(5))
+            same_maps(maps[i], m)
+            same_images(b.vmaps[i], vm)
+            same_images(b.nmaps[i][:-1, :-1], nm[:-1, :-1])
+            assert maps[i].stats()["sum_visible"] == m.stats()["sum_visible"]
+        b.close()
+
+
 def test_batch_scene_against_oracle(oracle_threads):
     import torch
     ws = scenes_spec()[1:2]

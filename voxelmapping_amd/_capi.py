@@ -100,6 +100,8 @@ class BatchTuning(C.Structure):
         ("visible_wg_per_cu", C.c_int32),
         ("integrate_wg_per_cu", C.c_int32),
         ("scene_groups", C.c_int32),
+        ("persistent", C.c_int32),
+        ("persist_wg_per_cu", C.c_int32),
     ]
 
 

@@ -51,12 +51,27 @@ struct BatchTuning {
     int visibleWgPerCu;     // visibility workgroups per compute unit
     int integrateWgPerCu;   // integration workgroups (4 waves each) per CU
     int sceneGroups;        // scenes split into this many groups, each on its own stream
+    int persistent;         // 0: five kernels; 1: one kernel with grid barriers; 2: the same, cooperative launch
+    int persistWgPerCu;     // resident workgroups per CU of the one-kernel step (capped by occupancy)
 };
+
+// device words of the one-kernel step
+struct PersistCtl {
+    int* status;            // bit 0: a grid barrier wait ran out
+    unsigned* arrivals;     // grid-barrier arrivals, never reset
+    unsigned* seq;          // launches run (the barrier targets' base)
+    unsigned* queues;       // ray-tile counters, one per XCD band, kQueueStride apart
+};
+constexpr int kQueueStride = 32;  // one 128-byte line per counter
 
 // the five stage kernels of one step over the scenes of B; with advance, then
 // the frame counter increment
 hipError_t batch_launch_step(const BatchView& B, const BatchTuning& t, int fmt, int df, bool color, hipStream_t s,
                              bool advance = true);
+// the same step as one kernel (stages separated by grid barriers), which also
+// advances the frame counter; needs one scene group
+hipError_t batch_launch_persistent(const BatchView& B, const PersistCtl& P, const BatchTuning& t, int fmt, int df,
+                                   bool color, hipStream_t s);
 hipError_t batch_advance(int* frame, hipStream_t s);
 hipError_t batch_set_frame(int* frame, int v, hipStream_t s);
 int batch_max_scenes();

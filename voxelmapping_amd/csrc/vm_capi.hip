@@ -871,6 +871,8 @@ int vm_map_upload_blocks(vm_map* m, int32_t first_block, int32_t n_blocks, const
 // ---------------------------------------------------------------------------
 // multi-scene batches
 
+constexpr int kBatchCtlInts = 512;
+
 struct vm_batch {
     std::vector<vm_map*> maps;
     std::vector<vm_batch_scene> desc;
@@ -878,12 +880,14 @@ struct vm_batch {
     BatchScene* dScenes = nullptr;
     Affine* dPoses = nullptr;  // per scene: nFrames camToWorld, then nFrames worldToCam
     unsigned long long* dRayKeys = nullptr;  // per scene: 2 (tiles + 1) ray-bound keys
-    int* dFrame = nullptr;  // [0] frame index, [1] steps run (ray-bound epoch - 1)
+    int* dFrame = nullptr;  // [0] frame index, [1] steps run (ray-bound epoch - 1); one-kernel step:
+                            // [2] status, [32] grid-barrier arrivals, [64] launches run,
+                            // [128 + 32 k] ray-tile counters (k < 8)
     std::vector<size_t> keyOff;     // first key of each scene in dRayKeys
     std::vector<size_t> keyHalf;    // near (= far) keys of each scene: tiles + 1
     uint64_t hostSteps = 0;         // steps issued since the keys were last filled
     std::vector<BatchView> groups;  // one view per scene group (contiguous in dScenes)
-    BatchTuning tuning{8, 1, 4, 8, 1};
+    BatchTuning tuning{8, 1, 4, 8, 1, 0, 4};
     int fmt = 0, df = 0;
     bool color = false;
     hipGraph_t graph = nullptr;
@@ -956,11 +960,18 @@ int batch_upload(vm_batch* b) {
     return VM_OK;
 }
 
+PersistCtl batch_ctl(vm_batch* b) {
+    return PersistCtl{b->dFrame + 2, (unsigned*)(b->dFrame + 32), (unsigned*)(b->dFrame + 64),
+                      (unsigned*)(b->dFrame + 128)};
+}
+
 // one step: group 0 on s, group g on side stream g - 1, forked from and
 // joined back into s by events (parallel branches when captured), then the
 // frame counter increment on s
 hipError_t batch_launch_all(vm_batch* b, hipStream_t s) {
     const int G = (int)b->groups.size();
+    if (b->tuning.persistent)
+        return batch_launch_persistent(b->groups[0], batch_ctl(b), b->tuning, b->fmt, b->df, b->color, s);
     if (G == 1) return batch_launch_step(b->groups[0], b->tuning, b->fmt, b->df, b->color, s);
     hipError_t e = hipSuccess;
     if (!b->fork) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
@@ -1086,9 +1097,9 @@ int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
     hipError_t e = hipMalloc((void**)&b->dPoses, sizeof(Affine) * nPoses);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dRayKeys, sizeof(unsigned long long) * nKeys);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dScenes, sizeof(BatchScene) * n);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->dFrame, sizeof(int) * 64);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dFrame, sizeof(int) * kBatchCtlInts);
     if (e == hipSuccess) e = hipMemcpy(b->dPoses, poses.data(), sizeof(Affine) * nPoses, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(b->dFrame, 0, sizeof(int) * 64);
+    if (e == hipSuccess) e = hipMemset(b->dFrame, 0, sizeof(int) * kBatchCtlInts);
     if (e == hipSuccess) e = batch_reset_keys(b, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
@@ -1157,8 +1168,13 @@ int vm_batch_set_frame(vm_batch* b, int32_t frame, void* stream) {
 int vm_batch_get_frame(vm_batch* b, void* stream, int32_t* frame) {
     if (!b || !frame) return fail(VM_ERR_INVALID_ARG, "batch and frame must not be NULL");
     hipStream_t s = (hipStream_t)stream;
-    VM_HIP(hipMemcpyAsync(frame, b->dFrame, sizeof(int), hipMemcpyDeviceToHost, s), "frame copy");
+    int w[3];
+    VM_HIP(hipMemcpyAsync(w, b->dFrame, sizeof(w), hipMemcpyDeviceToHost, s), "frame copy");
     VM_HIP(hipStreamSynchronize(s), "stream synchronize");
+    *frame = w[0];
+    if (w[2])
+        return fail(VM_ERR_STATE, "a one-kernel step's grid barrier wait ran out (its workgroups were not all "
+                                  "resident): that step's results are invalid");
     return VM_OK;
 }
 
@@ -1177,6 +1193,8 @@ int vm_batch_step(vm_batch* b, void* stream) {
 int vm_batch_capture(vm_batch* b, void* stream) {
     if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
     if (!stream) return fail(VM_ERR_INVALID_ARG, "capture needs a non-null stream");
+    if (b->tuning.persistent == 2)
+        return fail(VM_ERR_STATE, "a cooperative launch is not captured: use persistent 1 for a graph");
     hipStream_t s = (hipStream_t)stream;
     batch_drop_graph(b);
     VM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -1200,6 +1218,8 @@ int vm_batch_get_tuning(vm_batch* b, vm_batch_tuning* out) {
     out->visible_wg_per_cu = b->tuning.visibleWgPerCu;
     out->integrate_wg_per_cu = b->tuning.integrateWgPerCu;
     out->scene_groups = b->tuning.sceneGroups;
+    out->persistent = b->tuning.persistent;
+    out->persist_wg_per_cu = b->tuning.persistWgPerCu;
     return VM_OK;
 }
 
@@ -1211,15 +1231,24 @@ int vm_batch_set_tuning(vm_batch* b, const vm_batch_tuning* t) {
         t->integrate_wg_per_cu > 64)
         return fail(VM_ERR_INVALID_ARG, "workgroups per CU must be in [1, 64]");
     if (t->scene_groups < 1 || t->scene_groups > 4) return fail(VM_ERR_INVALID_ARG, "scene_groups must be in [1, 4]");
+    if (t->persistent < 0 || t->persistent > 2) return fail(VM_ERR_INVALID_ARG, "persistent must be 0, 1 or 2");
+    if (t->persistent && t->scene_groups != 1)
+        return fail(VM_ERR_INVALID_ARG, "the one-kernel step runs one scene group");
+    if (t->persist_wg_per_cu < 1 || t->persist_wg_per_cu > 32)
+        return fail(VM_ERR_INVALID_ARG, "persist_wg_per_cu must be in [1, 32]");
     VM_HIP(hipDeviceSynchronize(), "synchronize");  // steps in flight read the scene array
     batch_drop_graph(b);
     b->tuning = BatchTuning{t->ray_tile_width, t->xcd_bands ? 1 : 0, t->visible_wg_per_cu, t->integrate_wg_per_cu,
-                            t->scene_groups};
+                            t->scene_groups,   t->persistent,        t->persist_wg_per_cu};
+    // the grid-barrier targets assume one grid size since the count was last cleared
+    VM_HIP(hipMemset(b->dFrame + 32, 0, sizeof(int) * (kBatchCtlInts - 32)), "barrier clear");
     return batch_upload(b);
 }
 
 // Autotuner: time every candidate launch shape over `steps` consecutive
-// steps (which fuse frames, like any step) and keep the fastest.
+// steps (which fuse frames, like any step) and keep the fastest: 18 launch
+// shapes with one scene group, 2..4 groups with the best shape, then the
+// one-kernel step (22 trials with 4 or more scenes).
 int vm_batch_tune(vm_batch* b, void* stream, int32_t steps, vm_batch_tuning* best_out) {
     if (!b) return fail(VM_ERR_INVALID_ARG, "batch is NULL");
     if (steps < 1) return fail(VM_ERR_INVALID_ARG, "steps must be >= 1");
@@ -1227,7 +1256,7 @@ int vm_batch_tune(vm_batch* b, void* stream, int32_t steps, vm_batch_tuning* bes
     hipEvent_t e0, e1;
     VM_HIP(hipEventCreate(&e0), "event");
     VM_HIP(hipEventCreate(&e1), "event");
-    vm_batch_tuning best{8, 1, 4, 8, 1};
+    vm_batch_tuning best{8, 1, 4, 8, 1, 0, 4};
     float bestMs = 1e30f;
     const int tws[] = {8, 16, 32};
     const int xcds[] = {1, 0};
@@ -1251,10 +1280,17 @@ int vm_batch_tune(vm_batch* b, void* stream, int32_t steps, vm_batch_tuning* bes
     for (int tw : tws)
         for (int xb : xcds)
             for (int iw : ints)
-                if ((rc = trial(vm_batch_tuning{tw, xb, 4, iw, 1}))) goto done;
+                if ((rc = trial(vm_batch_tuning{tw, xb, 4, iw, 1, 0, 4}))) goto done;
     for (int g = 2; g <= 4 && g <= (int)b->maps.size(); ++g) {
         vm_batch_tuning t = best;
         t.scene_groups = g;
+        if ((rc = trial(t))) goto done;
+    }
+    {  // the one-kernel step with the best ray-tile shape
+        vm_batch_tuning t = best;
+        t.scene_groups = 1;
+        t.persistent = 1;
+        t.persist_wg_per_cu = 4;
         if ((rc = trial(t))) goto done;
     }
     rc = vm_batch_set_tuning(b, &best);

@@ -155,7 +155,7 @@ __device__ inline float load_depth(const DepthImage& d, int x, int y) {
 constexpr int kLaneKeys = 4;
 
 template <int DF>
-__device__ inline void allocate_pixel(const MapView& m, const DepthImage& d, const Intrinsics& K, const Affine& T,
+__device__ __forceinline__ void allocate_pixel(const MapView& m, const DepthImage& d, const Intrinsics& K, const Affine& T,
                                       int x, int y, bool inImage) {
     const int lane = threadIdx.x & 63;
     int kx[kLaneKeys], ky[kLaneKeys], kz[kLaneKeys];
@@ -249,7 +249,7 @@ __device__ inline bool block_visible(const MapView& m, int bx, int by, int bz, c
 // the eight voxels of the lane's column and the eight colour pixels
 // together. Voxels whose depth test fails are read but never written.
 template <int FMT, int DF, bool COLOR>
-__device__ inline void integrate_block(const MapView& m, const DepthImage& d, const ColorImage& c,
+__device__ __forceinline__ void integrate_block(const MapView& m, const DepthImage& d, const ColorImage& c,
                                        const Intrinsics& K, const Affine& Tinv, int blk, int4 key, int lane) {
     const int lx = lane & 7, ly = lane >> 3;
     const float vs = m.voxelSize, tr = m.trunc;
@@ -391,7 +391,7 @@ struct VisibleShared {
 // block (one per lane) and compact the visible ones with wave ballots into
 // the map's visible table (one atomic per chunk). Returns the chunk's visible
 // count. Must be called by all 256 threads of the workgroup.
-__device__ inline int visible_chunk(const MapView& m, const Intrinsics& K, const Affine& Tinv, int cols, int rows,
+__device__ __forceinline__ int visible_chunk(const MapView& m, const Intrinsics& K, const Affine& Tinv, int cols, int rows,
                                     int chunk, int A, VisibleShared& sh) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = chunk + threadIdx.x;

@@ -465,8 +465,12 @@ __device__ inline void ray_skip(Ray& r, float nearLim) {
 // before nearLim are skipped, and a ray that reaches farLim can only read
 // (1.0, invalid) from there to distEnd, which ends in a miss, so the march
 // stops there.
+// (The per-pixel and per-block functions of these headers are force-inlined:
+// in a file whose kernels call one of them several times the compiler
+// otherwise calls it out of line, with a stack frame, which cost the batched
+// raycast a quarter of its time.)
 template <int FMT, int RUNS = kRunsNone>
-__device__ inline void raycast_pixel(const MapView& m, float* out, int x, int y, const Intrinsics& K,
+__device__ __forceinline__ void raycast_pixel(const MapView& m, float* out, int x, int y, const Intrinsics& K,
                                      const Affine& pose, const Affine& Tinv, float vsInv, float rtStep,
                                      float nearLim = 0.0f, float farLim = INFINITY) {
     Ray r;
@@ -578,7 +582,7 @@ struct BoundsShared {
 // The same for blocks whose positions the caller has loaded: b is this
 // thread's block (x = kEmptyKey for none).
 template <int CH>
-__device__ inline void ray_bounds_blocks(const int4 b, const RayBounds& rb, int W, int H, const Intrinsics& K,
+__device__ __forceinline__ void ray_bounds_blocks(const int4 b, const RayBounds& rb, int W, int H, const Intrinsics& K,
                                          const Affine& Tinv, float3 c, float vs, float grow, BoundsShared<CH>& sh) {
     const int tid = threadIdx.x;
     const int tilesX = (W + 7) >> 3;
@@ -674,7 +678,7 @@ __device__ inline float3 normalized3(float3 a) {
 __device__ inline float3 cross3(float3 a, float3 b) {
     return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ inline void normal_pixel(const float* vmap, int64_t vpitch, float* nmap, int64_t npitch, int W, int H,
+__device__ __forceinline__ void normal_pixel(const float* vmap, int64_t vpitch, float* nmap, int64_t npitch, int W, int H,
                                     int x, int y) {
     const int x10 = max(x - 1, 0), x01 = min(x + 1, W);
     const int y10 = max(y - 1, 0), y01 = min(y + 1, H);
