@@ -210,7 +210,8 @@ int vm_surface_points(const float* verts, float* points, int64_t n_triangles, vo
  * The march is recomputed with its discrete decisions (cells, interpolation
  * test, break step) held fixed; memory is O(pixels). Float voxel format only.
  * Pixels whose march has more than 48 value-dependent steps contribute no
- * gradient and are counted in overflow_pixels. */
+ * gradient and are counted in overflow_pixels; the counts are kept only when
+ * out_stats is given (the call then waits for them). */
 typedef struct vm_backward_stats {
     int32_t hit_pixels;
     int32_t overflow_pixels;

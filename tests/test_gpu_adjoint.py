@@ -197,3 +197,39 @@ def test_autograd_function_matches_direct_call(scene):
     scale = gt.abs().max().item()
     assert torch.allclose(tsdf.grad, gt, rtol=1e-4, atol=1e-6 * scale)
     assert torch.allclose(xi.grad.double(), gp.cpu(), rtol=1e-5, atol=1e-6)
+
+
+def test_backward_reuses_forward_bounds(scene):
+    """The adjoint right after the forward raycast at the same pose reuses its
+    per-tile ray bounds; after a raycast at another pose it computes them
+    afresh. Both give the same pose gradient bit for bit (per-wave partial
+    sums, fixed reduction order), the same TSDF gradient up to the order of
+    float atomics, and the same counts; without stats the pose gradient is
+    unchanged too."""
+    import torch
+    w, g, o, Tf, vm = scene
+    hit = ~torch.isnan(vm[..., 0])
+    gv = torch.zeros_like(vm)
+    gv[..., 2] = hit.float()
+    gn = (torch.rand_like(vm) * 2 - 1) * hit[..., None].float()
+
+    def backward(stats=True):
+        gt = torch.zeros(g.desc.n_block * 512, dtype=torch.float32, device="cuda")
+        gp = torch.zeros(6, dtype=torch.float64, device="cuda")
+        st = g.raycast_backward(vm, K2, Tf, gt, gp, grad_vertex=gv, grad_normal=gn, stats=stats)
+        torch.cuda.synchronize()
+        return gt, gp, st
+
+    v2 = torch.zeros_like(vm)
+    g.raycast(v2, K2, Tf)  # same pose: same vertex map, bounds recorded
+    assert torch.equal(torch.nan_to_num(v2, 7.0), torch.nan_to_num(vm, 7.0))
+    gtA, gpA, stA = backward()
+    T2 = Tf.copy()
+    T2[0, 3] += 0.05  # 5 cm along x
+    g.raycast(v2, K2, T2)
+    gtB, gpB, stB = backward()
+    _, gpC, stC = backward(stats=False)
+    assert stA == stB and stA["hit_pixels"] > 100000 and stC is None
+    assert torch.equal(gpA, gpB) and torch.equal(gpA, gpC)
+    scale = gtA.abs().max().item()
+    assert scale > 0 and torch.allclose(gtA, gtB, rtol=1e-5, atol=1e-6 * scale)

@@ -206,6 +206,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 }
                 res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
                 dist_s += step;
+                // the unobserved samples after an unobserved one in the same
+                // block, read together: unit steps, sdf stays 1
+                if (!valid && p >= 0) units += run_samples(rd, res, dist_s, dir, distEnd, p, bx, by, bz, 1.0f, false);
             }
             if (depth > 0) stack[(int64_t)(depth - 1) * P + pix] = pend;
             if (sdf <= 0.0f && !overflow) {
@@ -291,20 +294,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         if (lane == 0) poseParts[(int64_t)blockIdx.x * 6 + k] = v;
     }
     const unsigned long long hits = __ballot(hit), ovf = __ballot(overflow);
-    if (lane == 0) {
+    if (lane == 0 && stats) {  // counted only when the caller asks
         if (hits) atomicAdd(&stats[0], __popcll(hits));
         if (ovf) atomicAdd(&stats[1], __popcll(ovf));
     }
 }
 
-__global__ void k_pose_reduce(const float* parts, int n, double* gPose) {
-    __shared__ double s[256];
+// the 6 pose components summed over the waves' parts in a fixed order (each
+// thread a strided slice in double, then a tree): deterministic
+constexpr int kReduceThreads = 1024;
+__global__ __launch_bounds__(kReduceThreads) void k_pose_reduce(const float* parts, int n, double* gPose) {
+    __shared__ double s[kReduceThreads];
     const int k = blockIdx.x;  // component
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += (double)parts[(int64_t)i * 6 + k];
+    for (int i = threadIdx.x; i < n; i += kReduceThreads) acc += (double)parts[(int64_t)i * 6 + k];
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = kReduceThreads / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
         __syncthreads();
     }
@@ -313,16 +319,54 @@ __global__ void k_pose_reduce(const float* parts, int n, double* gPose) {
 
 // Adjoint of computeNormalKernel gathered per vertex: g_v[q] = g_vertex[q] +
 // the contributions of every normal pixel whose stencil reads q. Normals
-// computed from a missed vertex (x = NaN) pass no gradient.
-__global__ void k_normals_backward(const float* vmap, int64_t vpitch, const float* gvin, int64_t gvpitch,
-                                   const float* gn, int64_t gnpitch, float* gout, int64_t gopitch, int W, int H) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= W || y >= H) return;
+// computed from a missed vertex (x = NaN) pass no gradient. A 16x16 tile of
+// vertices is reached by the normal pixels of the tile grown by one; each of
+// those 18x18 normal pixels' two stencil-difference gradients (ga for
+// v(x+1) - v(x-1), gb for v(y+1) - v(y-1)) is computed once into LDS, and each
+// vertex then gathers what its neighbours' stencils give it.
+constexpr int kNbT = 16, kNbH = kNbT + 2;
+__global__ __launch_bounds__(256) void k_normals_backward(const float* vmap, int64_t vpitch, const float* gvin,
+                                                          int64_t gvpitch, const float* gn, int64_t gnpitch,
+                                                          float* gout, int64_t gopitch, int W, int H) {
+    __shared__ float3 s_ga[kNbH * kNbH], s_gb[kNbH * kNbH];
+    __shared__ unsigned char s_on[kNbH * kNbH];
+    const int X0 = blockIdx.x * kNbT, Y0 = blockIdx.y * kNbT;
     auto V = [&](int yy, int xx) {
         const float4 v = *((const float4*)((const uint8_t*)vmap + (int64_t)yy * vpitch) + xx);
         return make_float3(v.x, v.y, v.z);
     };
+    if (gn) {
+        for (int t = threadIdx.x; t < kNbH * kNbH; t += 256) {
+            const int px = X0 - 1 + t % kNbH, py = Y0 - 1 + t / kNbH;
+            bool on = false;
+            if (px >= 0 && py >= 0 && px < W - 1 && py < H - 1) {  // the pixels computeNormal writes
+                const float4 g4 = *((const float4*)((const uint8_t*)gn + (int64_t)py * gnpitch) + px);
+                const float3 g = make_float3(g4.x, g4.y, g4.z);
+                if (g.x != 0.f || g.y != 0.f || g.z != 0.f) {
+                    const int x10 = max(px - 1, 0), x01 = min(px + 1, W), y10 = max(py - 1, 0), y01 = min(py + 1, H);
+                    const float3 v00 = V(py, x10), v01 = V(py, x01), v10 = V(y10, px), v11 = V(y01, px);
+                    if (!(isnan(v00.x) || isnan(v01.x) || isnan(v10.x) || isnan(v11.x))) {
+                        const float3 a = sub3(v01, v00), b = sub3(v11, v10);
+                        const float3 cr = crs3(a, b);
+                        const float n2 = sum3(cr.x * cr.x, cr.y * cr.y, cr.z * cr.z);
+                        float3 gc = g;
+                        if (n2 > 0.f) {
+                            const float nn = sqrtf(n2);
+                            const float3 nrm = mul3(cr, 1.0f / nn);
+                            gc = mul3(sub3(g, mul3(nrm, dot3f(nrm, g))), 1.0f / nn);
+                        }
+                        s_ga[t] = crs3(b, gc);
+                        s_gb[t] = crs3(gc, a);
+                        on = true;
+                    }
+                }
+            }
+            s_on[t] = on;
+        }
+    }
+    __syncthreads();
+    const int x = X0 + (threadIdx.x & 15), y = Y0 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
     float3 acc = make_float3(0.f, 0.f, 0.f);
     if (gvin) {
         const float4 g = *((const float4*)((const uint8_t*)gvin + (int64_t)y * gvpitch) + x);
@@ -333,23 +377,10 @@ __global__ void k_normals_backward(const float* vmap, int64_t vpitch, const floa
         const int cand[5][2] = {{x, y}, {x - 1, y}, {x + 1, y}, {x, y - 1}, {x, y + 1}};
         for (int c = 0; c < 5; ++c) {
             const int px = cand[c][0], py = cand[c][1];
-            if (px < 0 || py < 0 || px >= W - 1 || py >= H - 1) continue;
-            const float4 g4 = *((const float4*)((const uint8_t*)gn + (int64_t)py * gnpitch) + px);
-            const float3 g = make_float3(g4.x, g4.y, g4.z);
-            if (g.x == 0.f && g.y == 0.f && g.z == 0.f) continue;
+            const int t = (py - Y0 + 1) * kNbH + (px - X0 + 1);
+            if (!s_on[t]) continue;  // off the written pixels, no upstream gradient, or a missed vertex
             const int x10 = max(px - 1, 0), x01 = min(px + 1, W), y10 = max(py - 1, 0), y01 = min(py + 1, H);
-            const float3 v00 = V(py, x10), v01 = V(py, x01), v10 = V(y10, px), v11 = V(y01, px);
-            if (isnan(v00.x) || isnan(v01.x) || isnan(v10.x) || isnan(v11.x)) continue;
-            const float3 a = sub3(v01, v00), b = sub3(v11, v10);
-            const float3 cr = crs3(a, b);
-            const float n2 = sum3(cr.x * cr.x, cr.y * cr.y, cr.z * cr.z);
-            float3 gc = g;
-            if (n2 > 0.f) {
-                const float nn = sqrtf(n2);
-                const float3 nrm = mul3(cr, 1.0f / nn);
-                gc = mul3(sub3(g, mul3(nrm, dot3f(nrm, g))), 1.0f / nn);
-            }
-            const float3 ga = crs3(b, gc), gb = crs3(gc, a);
+            const float3 ga = s_ga[t], gb = s_gb[t];
             if (py == y && x01 == x) acc = add3(acc, ga);
             if (py == y && x10 == x) acc = sub3(acc, ga);
             if (px == x && y01 == y) acc = add3(acc, gb);
@@ -376,7 +407,8 @@ size_t raycast_backward_scratch_bytes(int W, int H) {
 hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t vpitch, const float* gVertex,
                                    int64_t gvpitch, const float* gNormal, int64_t gnpitch, int W, int H,
                                    const Intrinsics& K, const Affine& pose, const Affine& Tinv, float* gTsdf,
-                                   double* gPose, void* scratch, int* stats, const RayBounds& rb, hipStream_t s) {
+                                   double* gPose, void* scratch, int* stats, const RayBounds& rb,
+                                   bool launchBounds, hipStream_t s) {
     const int tilesX = (W + 7) / 8, nTiles = tilesX * ((H + 7) / 8);
     const int waves = (int)backward_waves(W, H);
     uint8_t* p = (uint8_t*)scratch;
@@ -386,16 +418,18 @@ hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t 
     p += parts_bytes(W, H);
     float* gv = (float*)p;
     const int64_t gpitch = (int64_t)W * 16;
-    hipLaunchKernelGGL(k_normals_backward, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, vmap, vpitch, gVertex,
-                       gvpitch, gNormal, gnpitch, gv, gpitch, W, H);
+    hipLaunchKernelGGL(k_normals_backward, dim3((W + kNbT - 1) / kNbT, (H + kNbT - 1) / kNbT), dim3(256), 0, s, vmap,
+                       vpitch, gVertex, gvpitch, gNormal, gnpitch, gv, gpitch, W, H);
     const float vsInv = (float)(1.0 / (double)m.voxelSize);
     const float rtStep = m.trunc / m.voxelSize;
     const float margin = ray_drift_margin(W, H, K, pose.t0, pose.t1, pose.t2, vsInv);
-    hipError_t e = launch_ray_bounds(m, rb, W, H, K, pose, Tinv, margin, s);
-    if (e != hipSuccess) return e;
+    if (launchBounds) {
+        hipError_t e = launch_ray_bounds(m, rb, W, H, K, pose, Tinv, margin, s);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_raycast_backward, dim3((unsigned)waves), dim3(64), 0, s, m, gv, gpitch, W, H, K, pose, Tinv,
                        vsInv, rtStep, gTsdf, parts, stack, stats, tilesX, waves / 8, nTiles, rb, margin);
-    hipLaunchKernelGGL(k_pose_reduce, dim3(6), dim3(256), 0, s, parts, waves, gPose);
+    hipLaunchKernelGGL(k_pose_reduce, dim3(6), dim3(kReduceThreads), 0, s, parts, waves, gPose);
     return hipGetLastError();
 }
 

@@ -112,6 +112,15 @@ struct vm_map {
     unsigned long long* rayKeys = nullptr;
     int rayKeysTiles = 0;
     unsigned rayEpoch = 0;
+    // the bounds of the last raycast: the adjoint reuses them while the
+    // allocated set (epoch), image size, intrinsics and pose are the same and
+    // no bounds were computed since (rayEpoch)
+    bool rbValid = false;
+    uint64_t rbMapEpoch = 0;
+    unsigned rbEpoch = 0;
+    int rbW = 0, rbH = 0;
+    float rbK[4];
+    float rbT[16];
     // work buffer of the mesh extraction (neighbour offsets and triangle
     // counts, about 9 ints per pool block), allocated by the first polygonize
     int* meshWork = nullptr;
@@ -131,6 +140,7 @@ int check_map(const vm_map* m) {
 // reads as "no block" for every epoch.
 int ray_bounds_for(vm_map* m, int W, int H, hipStream_t s, RayBounds* out) {
     const int n = ray_bounds_tiles(W, H);
+    m->rbValid = false;  // the caller records the bounds it launches
     if (n > m->rayKeysTiles || m->rayEpoch == 0xfffffffeu) {
         const int cap = std::max(n, 1 << 16);
         if (cap > m->rayKeysTiles) {
@@ -156,6 +166,21 @@ int ray_bounds_for(vm_map* m, int W, int H, hipStream_t s, RayBounds* out) {
     // epoch, which every newer key replaces
     *out = RayBounds{m->rayKeys, m->rayKeysTiles, m->rayEpoch};
     return VM_OK;
+}
+
+void remember_bounds(vm_map* m, int W, int H, const float K[4], const float T[16]) {
+    m->rbValid = true;
+    m->rbMapEpoch = m->epoch;
+    m->rbEpoch = m->rayEpoch;
+    m->rbW = W;
+    m->rbH = H;
+    std::memcpy(m->rbK, K, sizeof(m->rbK));
+    std::memcpy(m->rbT, T, sizeof(m->rbT));
+}
+
+bool bounds_current(const vm_map* m, int W, int H, const float K[4], const float T[16]) {
+    return m->rbValid && m->rbMapEpoch == m->epoch && m->rbEpoch == m->rayEpoch && m->rbW == W && m->rbH == H &&
+           !std::memcmp(m->rbK, K, sizeof(m->rbK)) && !std::memcmp(m->rbT, T, sizeof(m->rbT));
 }
 
 // The rotation block of a column-major camToWorld is orthonormal: every
@@ -466,9 +491,11 @@ int vm_raycast(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H, cons
     if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
     RayBounds rb;
     if (int rc = ray_bounds_for(m, W, H, (hipStream_t)stream, &rb)) return rc;
+    const bool rigid = rigid_rotation(T);
     VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, intrinsics_of(K), affine_of(T), affine_of(Ti), false,
-                          (hipStream_t)stream, rigid_rotation(T) ? &rb : nullptr),
+                          (hipStream_t)stream, rigid ? &rb : nullptr),
            "raycast");
+    if (rigid) remember_bounds(m, W, H, K, T);
     return VM_OK;
 }
 
@@ -488,9 +515,10 @@ int vm_render_scene(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H,
     }
     RayBounds rb;
     if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
-    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s,
-                          rigid_rotation(T) ? &rb : nullptr),
+    const bool rigid = rigid_rotation(T);
+    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s, rigid ? &rb : nullptr),
            "raycast");
+    if (rigid) remember_bounds(m, W, H, K, T);
     return VM_OK;
 }
 
@@ -636,14 +664,25 @@ int vm_raycast_backward(vm_map* m, const float* vmap, int64_t vpitch, const floa
         VM_HIP(hipMalloc(&m->scratch, need), "scratch allocation");
         m->scratchBytes = need;
     }
-    int* stats = (int*)((uint8_t*)m->scratch + need - 256);
-    VM_HIP(launch_fill_u32(stats, 0u, 2, s), "stats reset");
-    RayBounds rb;
-    if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
-    if (!rigid_rotation(T)) rb.keys = nullptr;  // no bounds: the forward raycast marches in full too
+    int* stats = out_stats ? (int*)((uint8_t*)m->scratch + need - 256) : nullptr;
+    if (stats) VM_HIP(launch_fill_u32(stats, 0u, 2, s), "stats reset");
+    // the bounds of the forward raycast at this pose when nothing has changed
+    // since, else fresh ones (none for a non-rigid pose: the forward raycast
+    // marches in full too)
+    RayBounds rb{nullptr, 0, 0u};
+    const bool rigid = rigid_rotation(T);
+    const bool reuse = rigid && bounds_current(m, W, H, K, T);
+    if (reuse) {
+        rb = RayBounds{m->rayKeys, m->rayKeysTiles, m->rbEpoch};
+    } else {
+        if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
+        if (!rigid) rb.keys = nullptr;
+    }
     VM_HIP(launch_raycast_backward(m->view, vmap, vpitch, gv, gvpitch, gn, gnpitch, W, H, intrinsics_of(K),
-                                   affine_of(T), affine_of(Ti), grad_tsdf, grad_pose, m->scratch, stats, rb, s),
+                                   affine_of(T), affine_of(Ti), grad_tsdf, grad_pose, m->scratch, stats, rb,
+                                   !reuse, s),
            "raycast backward");
+    if (rigid && !reuse) remember_bounds(m, W, H, K, T);
     if (out_stats) {
         int st[2];
         VM_HIP(hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, s), "stats copy");

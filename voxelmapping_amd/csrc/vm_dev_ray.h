@@ -327,7 +327,8 @@ __device__ inline void ray_begin(Ray& r, int x, int y, const Intrinsics& K, cons
 
 enum RayStatus : int { kRayGoing = 0, kRaySurface = 1, kRayMiss = 2 };
 
-// Runs of predictable samples (forward march only). A sample in an
+// Runs of predictable samples (the forward march, and the raycast adjoint's
+// recompute of it: unit steps only). A sample in an
 // allocated block that reads "invalid" (an unobserved voxel, or an
 // interpolation with an unobserved corner) steps exactly one voxel, and on
 // the rays that take longest it is mostly followed by more of them: rays
@@ -340,29 +341,33 @@ enum RayStatus : int { kRayGoing = 0, kRaySurface = 1, kRayMiss = 2 };
 // resumes at the first sample that reads anything else, the block's end or
 // distEnd. One memory round trip then covers up to kRunSpec samples that the
 // loop reads one after the other. Positions, distances and results are the
-// loop's own, bit for bit. Which runs are taken is a template choice of
-// ray_step: kRunsNone, kRunsInvalid (unit steps), kRunsAll (also 1.0 steps).
+// loop's own, bit for bit. Which runs ray_step takes is a template choice:
+// kRunsNone, kRunsInvalid (unit steps), kRunsAll (also 1.0 steps).
 enum RunKinds : int { kRunsNone = 0, kRunsInvalid = 1, kRunsAll = 2 };
 constexpr int kRunSpec = 4;
 
-// r is just past a sample of voxel block (bx, by, bz) (pool offset p) that
-// read "invalid" (valid = false, step 1) or a valid 1.0 (step rtStep), and
-// took its step `step`.
+// A march position res (distance dist) is just past a sample of voxel block
+// (bx, by, bz) (pool offset p) that read "invalid" (valid = false, step 1)
+// or a valid 1.0 (step rtStep), and took its step `step`. Up to kRunSpec
+// further samples of the loop lie in the same block; their voxels are read
+// together, and the leading ones that read the same (and so take the same
+// step, with nothing else changing) are taken: res and dist advance by the
+// same additions as the loop's. Returns the number of samples taken.
 template <int FMT>
-__device__ inline void ray_run(Ray& r, const RayReader<FMT>& rd, int p, int bx, int by, int bz, float step,
-                               bool valid) {
+__device__ inline int run_samples(const RayReader<FMT>& rd, float3& res, float& dist, float3 dir, float distEnd,
+                                  int p, int bx, int by, int bz, float step, bool valid) {
     int loc[kRunSpec];
     int n = 0;
-    float3 q = r.res;
-    float d = r.dist_s;
+    float3 q = res;
+    float d = dist;
 #pragma unroll
     for (int k = 0; k < kRunSpec; ++k) {
         loc[k] = 0;
-        if (n == k && d < r.distEnd) {  // the loop condition of sample k
+        if (n == k && d < distEnd) {  // the loop condition of sample k
             const int qx = floor_i(q.x), qy = floor_i(q.y), qz = floor_i(q.z);
             if (vox_to_blk(qx) == bx && vox_to_blk(qy) == by && vox_to_blk(qz) == bz) {
                 loc[k] = local_index(vox_to_loc(qx), vox_to_loc(qy), vox_to_loc(qz));
-                q = make_float3(q.x + step * r.dir.x, q.y + step * r.dir.y, q.z + step * r.dir.z);
+                q = make_float3(q.x + step * dir.x, q.y + step * dir.y, q.z + step * dir.z);
                 d += step;
                 n = k + 1;
             }
@@ -381,9 +386,16 @@ __device__ inline void ray_run(Ray& r, const RayReader<FMT>& rd, int p, int bx, 
     for (int k = 0; k < kRunSpec; ++k)
         if (acc == k && k < n && (valid ? (ok[k] && sv[k] == 1.0f) : !ok[k])) acc = k + 1;
     for (int k = 0; k < acc; ++k) {
-        r.res = make_float3(r.res.x + step * r.dir.x, r.res.y + step * r.dir.y, r.res.z + step * r.dir.z);
-        r.dist_s += step;
+        res = make_float3(res.x + step * dir.x, res.y + step * dir.y, res.z + step * dir.z);
+        dist += step;
     }
+    return acc;
+}
+
+template <int FMT>
+__device__ inline void ray_run(Ray& r, const RayReader<FMT>& rd, int p, int bx, int by, int bz, float step,
+                               bool valid) {
+    run_samples(rd, r.res, r.dist_s, r.dir, r.distEnd, p, bx, by, bz, step, valid);
 }
 
 // One pass of the march loop body (ray_tracing.cu:355-379): a sample, or a

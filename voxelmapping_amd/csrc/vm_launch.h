@@ -57,13 +57,15 @@ hipError_t launch_polygonize(const MapView& m, int* work, int capBlocks, float* 
 hipError_t launch_surface_points(const float* verts, float* points, int64_t n, hipStream_t s);
 
 // adjoint of the raycast + normals (float voxel format); scratch must hold
-// raycast_backward_scratch_bytes(W, H); stats = {hit pixels, overflowed pixels}
+// raycast_backward_scratch_bytes(W, H); stats = {hit pixels, overflowed
+// pixels} or null; launchBounds: compute rb's bounds first (false when rb's
+// epoch already holds them for this pose)
 size_t raycast_backward_scratch_bytes(int W, int H);
 hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t vpitch, const float* gVertex,
                                    int64_t gvpitch, const float* gNormal, int64_t gnpitch, int W, int H,
                                    const Intrinsics& K, const Affine& camToWorld, const Affine& worldToCam,
                                    float* gTsdf, double* gPose, void* scratch, int* stats, const RayBounds& rb,
-                                   hipStream_t s);
+                                   bool launchBounds, hipStream_t s);
 // the per-tile ray-distance bounds of rb's epoch for a raycast at this pose;
 // grow = ray_drift_margin of the pose (vm_dev_ray.h)
 hipError_t launch_ray_bounds(const MapView& m, const RayBounds& rb, int W, int H, const Intrinsics& K,
