@@ -59,7 +59,8 @@ def test_loader_parses_and_associates(tmp_path):
     for i in range(n):
         d, c, t, T = L.GetNext()
         assert d.dtype == np.uint16 and np.array_equal(d, depths[i])
-        assert c.dtype == np.uint8 and np.array_equal(c, colors[i])
+        # cv::imread's channel order: blue, green, red
+        assert c.dtype == np.uint8 and np.array_equal(c, colors[i][..., ::-1])
         assert t == times[i]
         dist = np.abs(np.array(gt_times) - t)
         best = int(np.flatnonzero(dist == dist.min())[0])
@@ -68,6 +69,18 @@ def test_loader_parses_and_associates(tmp_path):
     assert np.array_equal(L.getFirstFramePose(), L.groundTruth[0])
     # depth scale of the example: 5000 units per metre
     assert TUM_DEPTH_SCALE == 1.0 / 5000.0
+
+
+def test_loader_colour_order(tmp_path):
+    colors = [np.array([[[255, 0, 0], [0, 255, 0]], [[0, 0, 255], [10, 20, 30]]], np.uint8)]
+    write_sequence(tmp_path, [np.ones((2, 2), np.uint16)], colors, [5.0], [(5.0, 0, 0, 0, 0, 0, 0, 1)])
+    for order, want in (("bgr", colors[0][..., ::-1]), ("rgb", colors[0])):
+        L = DatasetLoader(str(tmp_path), color_order=order)
+        L.loadImages(True)
+        _, c, _, _ = L.GetNext()
+        assert np.array_equal(c, want) and c.flags["C_CONTIGUOUS"]
+    with pytest.raises(ValueError):
+        DatasetLoader(str(tmp_path), color_order="hsv")
 
 
 def test_loader_missing_files_raise(tmp_path):

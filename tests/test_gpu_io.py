@@ -10,10 +10,19 @@ from parity import colmajor, compare_maps, fuse_both, make_pair
 pytestmark = pytest.mark.gpu
 
 
-def test_map_save_load_round_trip(tmp_path, oracle_threads):
+def _mesh(m, cap=600_000):
+    import torch
+    verts = torch.zeros((cap, 9), dtype=torch.float32, device="cuda")
+    norms = torch.zeros_like(verts)
+    _, t = m.polygonize(verts, norms)
+    return verts[:t].cpu().numpy(), norms[:t].cpu().numpy()
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_map_save_load_round_trip(tmp_path, oracle_threads, fmt):
     import torch
     from voxelmapping_amd import DeviceMap, synth
-    w = synth.room_small(seed=2, voxel_format=1, n_frames=6)
+    w = synth.room_small(seed=2, voxel_format=fmt, n_frames=6)
     g, o = make_pair(w)
     fuse_both(w, g, o, range(4))
     path = str(tmp_path / "room.npz")
@@ -31,9 +40,28 @@ def test_map_save_load_round_trip(tmp_path, oracle_threads):
     h.raycast(vb, w.K, T)
     torch.cuda.synchronize()
     assert torch.equal(torch.nan_to_num(va, 7.0), torch.nan_to_num(vb, 7.0))
+    # the same mesh from the loaded map
+    mg, mh = _mesh(g), _mesh(h)
+    assert len(mg[0]) > 1000 and np.array_equal(mg[0], mh[0]) and np.array_equal(mg[1], mh[1])
+    # voxels written from the host (a range of blocks loses its weights):
+    # the mesh follows them, and equals that of the saved and reloaded map
+    k = h.stats()["allocated_blocks"] // 3
+    vox = h.download_blocks(0, k).reshape(k * 512, -1).copy()
+    if fmt == 2:
+        vox[:, 4:8] = 0  # float weight
+    else:
+        vox[:, 2] = 0  # weight byte
+    h.upload_blocks(0, vox)
+    path2 = str(tmp_path / "room2.npz")
+    h.save(path2)
+    h2 = DeviceMap.load(path2)
+    m1, m2 = _mesh(h), _mesh(h2)
+    assert 0 < len(m1[0]) < len(mg[0]) and np.array_equal(m1[0], m2[0]) and np.array_equal(m1[1], m2[1])
+    h2.release()
     # the loaded map keeps fusing exactly as the oracle does
+    h = DeviceMap.load(path)
     fuse_both(w, h, o, range(4, 6))
-    compare_maps(h, o, 1)
+    compare_maps(h, o, fmt)
 
 
 def test_tum_sequence_from_disk(tmp_path, oracle_threads):

@@ -24,7 +24,7 @@ import numpy as np
 TUM_DEPTH_SCALE = 1.0 / 5000.0
 
 
-def _read_png(path: str) -> np.ndarray:
+def _read_png(path: str, bgr: bool = False) -> np.ndarray:
     from PIL import Image  # decoding only; PIL is the image library present here
     with Image.open(path) as im:
         if im.mode in ("I;16", "I;16B", "I;16L"):
@@ -32,7 +32,8 @@ def _read_png(path: str) -> np.ndarray:
         if im.mode == "I":
             return np.array(im).astype(np.uint16)
         if im.mode in ("RGB", "RGBA", "L"):
-            return np.array(im.convert("RGB"), dtype=np.uint8)
+            c = np.array(im.convert("RGB"), dtype=np.uint8)
+            return np.ascontiguousarray(c[..., ::-1]) if bgr else c
         raise ValueError(f"{path}: unsupported PNG mode {im.mode}")
 
 
@@ -61,9 +62,18 @@ def _records(path: str):
 class DatasetLoader:
     """Mirror of vmap::DatasetLoader. ``GetNext()`` returns
     ``(depth_u16 HxW, color HxWx3 uint8, time, camToWorld 4x4 float64)`` or
-    None at the end; the pose is None when no ground truth was loaded."""
+    None at the end; the pose is None when no ground truth was loaded.
 
-    def __init__(self, filepath: str):
+    The colour comes in the reference's channel order by default: it reads
+    images with ``cv::imread(path, -1)`` (dataset_loader.cpp:131-132), which
+    gives blue, green, red; ``color_order="rgb"`` gives red, green, blue. An
+    alpha channel is dropped and a grey image is repeated into three channels
+    (the reference would keep both as they are), since fusion takes three."""
+
+    def __init__(self, filepath: str, color_order: str = "bgr"):
+        if color_order not in ("bgr", "rgb"):
+            raise ValueError("color_order must be 'bgr' or 'rgb'")
+        self.colorOrder = color_order
         self.rootPath = filepath if filepath.endswith("/") else filepath + "/"
         self.calibPath = self.rootPath + "calibration.txt"
         self.groundTruthPath = self.rootPath + "groundtruth.txt"
@@ -125,7 +135,7 @@ class DatasetLoader:
             return None
         i = self.currPos
         depth = _read_png(self.depthFilepath[i])
-        color = _read_png(self.colorFilepath[i])
+        color = _read_png(self.colorFilepath[i], bgr=self.colorOrder == "bgr")
         if depth.dtype != np.uint16 or depth.ndim != 2:
             raise ValueError(f"{self.depthFilepath[i]}: depth must be a 16-bit single-channel PNG")
         pose = self.groundTruth[i] if self.groundTruth else None
