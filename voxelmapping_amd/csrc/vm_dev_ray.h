@@ -111,34 +111,73 @@ struct RayReader {
         return v ? s : 1.0f;
     }
 
-    // read_sdf_interped, ray_tracing.cu:288-321. floor(q + e_k) is computed
-    // per axis exactly as the reference adds the unit offset before flooring.
-    __device__ inline float interp(float3 q, bool& valid) {
+    // read_sdf_interped, ray_tracing.cu:288-321, at q with floor(q) =
+    // (X0, Y0, Z0) in block (bx, by, bz) at pool offset p (-1: absent).
+    // floor(q + e_k) is computed per axis exactly as the reference adds the
+    // unit offset before flooring. Corner k's block differs from (bx, by, bz)
+    // only along the axes where floor(q + 1) crosses into the next block, so
+    // only those neighbours are resolved (one lookup for the usual single
+    // crossing, none for the two thirds of samples whose corners all lie in
+    // the block) instead of one cache lookup per corner. With HAVE0 the value
+    // of corner 0 (the voxel floor(q) itself) is the caller's s0 / ok0, the
+    // march's own sample of it.
+    template <bool HAVE0>
+    __device__ __forceinline__ float interp_in(float3 q, int X0, int Y0, int Z0, int bx, int by, int bz, int p,
+                                               float s0, bool ok0, bool& valid) {
         const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
-        const int X0 = floor_i(q.x), X1 = floor_i(q.x + 1.f);
-        const int Y0 = floor_i(q.y), Y1 = floor_i(q.y + 1.f);
-        const int Z0 = floor_i(q.z), Z1 = floor_i(q.z + 1.f);
-        int64_t idx[8];
-        idx[0] = voxel(X0, Y0, Z0);
-        idx[1] = voxel(X1, Y0, Z0);
-        idx[2] = voxel(X0, Y1, Z0);
-        idx[3] = voxel(X1, Y1, Z0);
-        idx[4] = voxel(X0, Y0, Z1);
-        idx[5] = voxel(X1, Y0, Z1);
-        idx[6] = voxel(X0, Y1, Z1);
-        idx[7] = voxel(X1, Y1, Z1);
+        const int X1 = floor_i(q.x + 1.f), Y1 = floor_i(q.y + 1.f), Z1 = floor_i(q.z + 1.f);
+        const int bx1 = vox_to_blk(X1), by1 = vox_to_blk(Y1), bz1 = vox_to_blk(Z1);
+        const bool sx = bx1 != bx, sy = by1 != by, sz = bz1 != bz;
+        // pb[c], c = i | j << 1 | l << 2: the block of the corners whose
+        // coordinates are X_i, Y_j, Z_l (the same block as c with a bit
+        // cleared where that axis does not cross)
+        int pb[8];
+        pb[0] = p;
+        pb[1] = sx ? block(bx1, by, bz) : p;
+        pb[2] = sy ? block(bx, by1, bz) : p;
+        pb[3] = sx && sy ? block(bx1, by1, bz) : sx ? pb[1] : pb[2];
+        pb[4] = sz ? block(bx, by, bz1) : p;
+        pb[5] = sx && sz ? block(bx1, by, bz1) : sx ? pb[1] : pb[4];
+        pb[6] = sy && sz ? block(bx, by1, bz1) : sy ? pb[2] : pb[4];
+        pb[7] = sx && sy && sz ? block(bx1, by1, bz1)
+                : sx && sy   ? pb[3]
+                : sx && sz   ? pb[5]
+                : sy && sz   ? pb[6]
+                : sx         ? pb[1]
+                : sy         ? pb[2]
+                : pb[4];
+        const int lx0 = vox_to_loc(X0), lx1 = vox_to_loc(X1);
+        const int ly0 = vox_to_loc(Y0), ly1 = vox_to_loc(Y1);
+        const int lz0 = vox_to_loc(Z0), lz1 = vox_to_loc(Z1);
         float s[8];
         bool all = true;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             float v = 1.0f;
             bool ok = false;
-            if (idx[k] >= 0) Vox<FMT>::read(m.voxels, idx[k], v, ok);
+            if (HAVE0 && k == 0) {
+                v = s0;
+                ok = ok0;
+            } else {
+                const int b = pb[k];
+                if (b >= 0)
+                    Vox<FMT>::read(m.voxels,
+                                   (int64_t)b + local_index((k & 1) ? lx1 : lx0, (k & 2) ? ly1 : ly0,
+                                                            (k & 4) ? lz1 : lz0),
+                                   v, ok);
+            }
             s[k] = ok ? v : 1.0f;
             all = all && ok;
         }
         valid = all;
         return blend(fx, fy, fz, s);
+    }
+
+    // read_sdf_interped at an arbitrary q
+    __device__ inline float interp(float3 q, bool& valid) {
+        const int X0 = floor_i(q.x), Y0 = floor_i(q.y), Z0 = floor_i(q.z);
+        const int bx = vox_to_blk(X0), by = vox_to_blk(Y0), bz = vox_to_blk(Z0);
+        return interp_in<false>(q, X0, Y0, Z0, bx, by, bz, block(bx, by, bz), 1.0f, false, valid);
     }
 
     // the trilinear blend of read_sdf_interped, ray_tracing.cu:305-320
@@ -158,7 +197,9 @@ struct RayReader {
     __device__ inline float sample(float3 q, int vx, int vy, int vz, int p, bool& valid) {
         const int lx = vox_to_loc(vx), ly = vox_to_loc(vy), lz = vox_to_loc(vz);
         float sdf = read_at(p < 0 ? -1 : (int64_t)p + local_index(lx, ly, lz), valid);
-        if (sdf <= 0.5f && sdf >= -0.5f) sdf = interp(q, valid);
+        if (sdf <= 0.5f && sdf >= -0.5f)
+            sdf = interp_in<true>(q, vx, vy, vz, vox_to_blk(vx), vox_to_blk(vy), vox_to_blk(vz), p, sdf, valid,
+                                  valid);
         return sdf;
     }
 };
@@ -221,69 +262,79 @@ __device__ inline void trilinear(RayReader<FMT>& rd, float3 q, Trilinear& t) {
 }
 
 // The march's samples in blocks that no hash entry holds (read_sdf gives
-// (1.0, invalid), the step is one voxel), starting at res in such a block
-// (bx, by, bz) whose bounds are exact floats. Runs the same additions as the
-// reference's loop and stops after the first sample outside the blocks known
-// to be empty, or at distEnd. Returns the number of unit steps taken.
-// inv = 1 / |dir| per axis.
+// (1.0, invalid), the step is one voxel). The position res lies in block
+// (cbx, cby, cbz), known to be absent, whose bounds are exact floats. One box
+// of the walk: the same additions as the reference's loop through the
+// largest box known to be empty around that block, ending with the first
+// sample outside it. Returns true while the walk can go on (the sample is
+// below distEnd, in a block known to be absent: (cbx, cby, cbz) is then that
+// block), false once the march must take the next sample itself. Adds the
+// unit steps taken to `steps`. inv = 1 / |dir| per axis.
 template <int FMT>
-__device__ inline int empty_walk(RayReader<FMT>& rd, float3& res, const float3 dir, const float3 inv, float& dist_s,
-                                 const float distEnd, const int bx, const int by, const int bz) {
+__device__ __forceinline__ bool walk_box(const RayReader<FMT>& rd, float3& res, const float3 dir, const float3 inv,
+                                         float& dist_s, const float distEnd, int& cbx, int& cby, int& cbz,
+                                         int& steps) {
     // The walk goes on across a face into the next block when that block's
     // bit in the brick word the lane holds is clear. The box known to be
     // empty is the largest of: the whole brick (32^3 voxels) when its word is
     // zero, the brick octant (2x2x2 blocks, bit mask 0x330033 shifted to the
     // octant) holding the current block when its eight bits are clear, the
     // current block.
-    int cbx = bx, cby = by, cbz = bz, steps = 0;
-    while (true) {
-        const bool held = rd.holds(cbx, cby, cbz);
-        const bool brick = held && rd.word == 0ull;
-        const bool oct = held && (rd.word & (0x330033ull << (((cbx >> 1) & 1) * 2 + ((cby >> 1) & 1) * 8 +
-                                                            ((cbz >> 1) & 1) * 32))) == 0ull;
-        const float edge = brick ? 32.f : oct ? 16.f : 8.f;
-        const int am = brick ? ~3 : oct ? ~1 : ~0;  // block coordinates of the box's low corner
-        const float lx = (float)((cbx & am) * 8), ly = (float)((cby & am) * 8), lz = (float)((cbz & am) * 8);
-        const float hx = lx + edge, hy = ly + edge, hz = lz + edge;
-        // Unit steps that stay strictly inside the box and below distEnd
-        // whatever the rounding of the additions: k <= 64 additions to
-        // coordinates below B = max(|l|, |h|) + 1 move them at most
-        // 64 * 2^-24 * B from the exact k * dir, which the margins cover. They
-        // are the same additions as the checked steps, without the checks.
-        {
-            const float ex = (fmaxf(fabsf(lx), fabsf(hx)) + 1.f) * 0x1p-18f + 0x1p-16f;
-            const float ey = (fmaxf(fabsf(ly), fabsf(hy)) + 1.f) * 0x1p-18f + 0x1p-16f;
-            const float ez = (fmaxf(fabsf(lz), fabsf(hz)) + 1.f) * 0x1p-18f + 0x1p-16f;
-            const float kx = ((dir.x > 0.f ? hx - res.x : res.x - lx) - ex) * inv.x;
-            const float ky = ((dir.y > 0.f ? hy - res.y : res.y - ly) - ey) * inv.y;
-            const float kz = ((dir.z > 0.f ? hz - res.z : res.z - lz) - ez) * inv.z;
-            // minimum that keeps a NaN (no unchecked steps then)
-            float q = fminf((distEnd - dist_s) - 2.0f, 64.0f);
-            q = kx >= q ? q : kx;
-            q = ky >= q ? q : ky;
-            q = kz >= q ? q : kz;
-            int n = q >= 1.0f ? (int)q : 0;
-            steps += n + 1;
-            for (; n >= 2; n -= 2) {
-                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                dist_s += 1.0f;
-                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                dist_s += 1.0f;
-            }
-            if (n) {
-                res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-                dist_s += 1.0f;
-            }
+    const bool held = rd.holds(cbx, cby, cbz);
+    const bool brick = held && rd.word == 0ull;
+    const bool oct = held && (rd.word & (0x330033ull << (((cbx >> 1) & 1) * 2 + ((cby >> 1) & 1) * 8 +
+                                                        ((cbz >> 1) & 1) * 32))) == 0ull;
+    const float edge = brick ? 32.f : oct ? 16.f : 8.f;
+    const int am = brick ? ~3 : oct ? ~1 : ~0;  // block coordinates of the box's low corner
+    const float lx = (float)((cbx & am) * 8), ly = (float)((cby & am) * 8), lz = (float)((cbz & am) * 8);
+    const float hx = lx + edge, hy = ly + edge, hz = lz + edge;
+    // Unit steps that stay strictly inside the box and below distEnd
+    // whatever the rounding of the additions: k <= 64 additions to
+    // coordinates below B = max(|l|, |h|) + 1 move them at most
+    // 64 * 2^-24 * B from the exact k * dir, which the margins cover. They
+    // are the same additions as the checked steps, without the checks.
+    {
+        const float ex = (fmaxf(fabsf(lx), fabsf(hx)) + 1.f) * 0x1p-18f + 0x1p-16f;
+        const float ey = (fmaxf(fabsf(ly), fabsf(hy)) + 1.f) * 0x1p-18f + 0x1p-16f;
+        const float ez = (fmaxf(fabsf(lz), fabsf(hz)) + 1.f) * 0x1p-18f + 0x1p-16f;
+        const float kx = ((dir.x > 0.f ? hx - res.x : res.x - lx) - ex) * inv.x;
+        const float ky = ((dir.y > 0.f ? hy - res.y : res.y - ly) - ey) * inv.y;
+        const float kz = ((dir.z > 0.f ? hz - res.z : res.z - lz) - ez) * inv.z;
+        // minimum that keeps a NaN (no unchecked steps then)
+        float q = fminf((distEnd - dist_s) - 2.0f, 64.0f);
+        q = kx >= q ? q : kx;
+        q = ky >= q ? q : ky;
+        q = kz >= q ? q : kz;
+        int n = q >= 1.0f ? (int)q : 0;
+        steps += n + 1;
+        for (; n >= 2; n -= 2) {
+            res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+            dist_s += 1.0f;
+            res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+            dist_s += 1.0f;
         }
-        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
-        dist_s += 1.0f;
-        if (!(dist_s < distEnd)) break;
-        if (res.x >= lx && res.x < hx && res.y >= ly && res.y < hy && res.z >= lz && res.z < hz) continue;
-        cbx = vox_to_blk(floor_i(res.x));
-        cby = vox_to_blk(floor_i(res.y));
-        cbz = vox_to_blk(floor_i(res.z));
-        if (!(exact_range(cbx) && exact_range(cby) && exact_range(cbz)) || !rd.known_absent(cbx, cby, cbz))
-            break;
+        if (n) {
+            res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+            dist_s += 1.0f;
+        }
+    }
+    res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+    dist_s += 1.0f;
+    if (!(dist_s < distEnd)) return false;
+    if (res.x >= lx && res.x < hx && res.y >= ly && res.y < hy && res.z >= lz && res.z < hz) return true;
+    cbx = vox_to_blk(floor_i(res.x));
+    cby = vox_to_blk(floor_i(res.y));
+    cbz = vox_to_blk(floor_i(res.z));
+    return exact_range(cbx) && exact_range(cby) && exact_range(cbz) && rd.known_absent(cbx, cby, cbz);
+}
+
+// The whole walk from block (bx, by, bz): boxes until the march must take a
+// sample itself, or distEnd. Returns the number of unit steps taken.
+template <int FMT>
+__device__ inline int empty_walk(RayReader<FMT>& rd, float3& res, const float3 dir, const float3 inv, float& dist_s,
+                                 const float distEnd, const int bx, const int by, const int bz) {
+    int cbx = bx, cby = by, cbz = bz, steps = 0;
+    while (walk_box(rd, res, dir, inv, dist_s, distEnd, cbx, cby, cbz, steps)) {
     }
     return steps;
 }
@@ -398,21 +449,38 @@ __device__ inline void ray_run(Ray& r, const RayReader<FMT>& rd, int p, int bx, 
     run_samples(rd, r.res, r.dist_s, r.dir, r.distEnd, p, bx, by, bz, step, valid);
 }
 
+// A walk through empty blocks that a lane has started and takes one box at
+// a time (walk_box): the block the position is in, and whether it is on.
+struct Walk {
+    int bx, by, bz, steps;
+    bool on;
+};
+
 // One pass of the march loop body (ray_tracing.cu:355-379): a sample, or a
 // run of samples in empty blocks. kRaySurface: the loop broke on sdf <= 0;
 // kRayMiss: it returned on a sign change or ran out of distance.
 // rtStep = trunc / voxelSize (ray_tracing.cu:431). RUNS (RunKinds): the
 // runs of samples taken after a sample that starts one (ray_run) -- for the
 // forward raycast only, as a run's samples are not visited one by one.
+// With a Walk, a walk through empty blocks is only started (wk->on, its
+// first block, the sample state it leaves: (1.0, invalid)) and the caller
+// takes its boxes; without one it is run here to its end.
 template <int FMT, int RUNS = kRunsNone>
-__device__ inline int ray_step(Ray& r, RayReader<FMT>& rd, float rtStep) {
+__device__ __forceinline__ int ray_step(Ray& r, RayReader<FMT>& rd, float rtStep, Walk* wk = nullptr) {
     if (!(r.dist_s < r.distEnd)) return kRayMiss;  // loop condition; sdf > 0 or NaN here
     const float last = r.sdf;
     const int vx = floor_i(r.res.x), vy = floor_i(r.res.y), vz = floor_i(r.res.z);
     const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
     const int p = rd.block(bx, by, bz);
     if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
-        empty_walk(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, bx, by, bz);
+        if (wk) {
+            wk->on = true;
+            wk->bx = bx;
+            wk->by = by;
+            wk->bz = bz;
+        } else {
+            empty_walk(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, bx, by, bz);
+        }
         r.sdf = 1.0f;
         r.valid = false;
         return kRayGoing;
@@ -490,9 +558,17 @@ __device__ __forceinline__ void raycast_pixel(const MapView& m, float* out, int 
     r.distEnd = fminf(r.distEnd, farLim);
     ray_skip(r, nearLim);
     RayReader<FMT> rd(m);
-    int st;
-    while ((st = ray_step<FMT, RUNS>(r, rd, rtStep)) == kRayGoing) {
-    }
+    // The loop takes, per pass, one box of a walk through empty blocks for
+    // the lanes that are walking and one march step for the others, so that
+    // the lanes of a wave walk side by side: a whole walk run at once by one
+    // lane is a loop of additions that its other lanes wait through, one
+    // lane after the other.
+    int st = kRayGoing;
+    Walk wk{0, 0, 0, 0, false};
+    do {
+        if (wk.on) wk.on = walk_box(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, wk.bx, wk.by, wk.bz, wk.steps);
+        if (!wk.on) st = ray_step<FMT, RUNS>(r, rd, rtStep, &wk);
+    } while (st == kRayGoing);
     float4 v;
     if (st == kRaySurface && ray_surface(r, rd, Tinv, vsInv, rtStep, v)) *(float4*)out = v;
     else out[0] = __int_as_float(0x7fffffff);
