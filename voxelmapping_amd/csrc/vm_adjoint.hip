@@ -165,13 +165,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                     ++units;
                 }
             }
-            while (dist_s < distEnd) {
+            // walks through empty blocks one box per pass, side by side with
+            // the other lanes' samples (march, vm_dev_ray.h)
+            Walk wk{0, 0, 0, 0, false};
+            while (true) {
+                if (wk.on) wk.on = walk_box(rd, res, dir, inv, dist_s, distEnd, wk.bx, wk.by, wk.bz, units);
+                if (wk.on) continue;
+                if (!(dist_s < distEnd)) break;
                 last = sdf;
                 const int vx = floor_i(res.x), vy = floor_i(res.y), vz = floor_i(res.z);
                 const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
                 const int p = rd.block(bx, by, bz);
                 if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
-                    units += empty_walk(rd, res, dir, inv, dist_s, distEnd, bx, by, bz);
+                    wk = Walk{bx, by, bz, 0, true};
                     sdf = 1.0f;
                     valid = false;
                     continue;

@@ -14,6 +14,7 @@
 
 #include "vm_batch.h"
 #include "vm_common.h"
+#include "vm_dev_ray.h"
 #include "vm_launch.h"
 #include "vmapping_capi.h"
 
@@ -181,6 +182,25 @@ void remember_bounds(vm_map* m, int W, int H, const float K[4], const float T[16
 bool bounds_current(const vm_map* m, int W, int H, const float K[4], const float T[16]) {
     return m->rbValid && m->rbMapEpoch == m->epoch && m->rbEpoch == m->rayEpoch && m->rbW == W && m->rbH == H &&
            !std::memcmp(m->rbK, K, sizeof(m->rbK)) && !std::memcmp(m->rbT, T, sizeof(m->rbT));
+}
+
+bool rigid_rotation(const float* T);
+
+// The visibility launch of a fusion also computes the ray bounds of a
+// raycast at the fusion's pose, image size and intrinsics (fresh epoch), so
+// that the raycast which usually follows at that view skips its own pass
+// over the allocated blocks (bounds_current then holds). Not for a pose
+// whose rotation is not orthonormal (the bounds assume it): *use false.
+int fusion_bounds(vm_map* m, int W, int H, const float K[4], const float T[16], hipStream_t s, BoundsAtPose* out,
+                  bool* use) {
+    *use = rigid_rotation(T);
+    if (!*use) return VM_OK;
+    if (int rc = ray_bounds_for(m, W, H, s, &out->rb)) return rc;
+    const float vsInv = (float)(1.0 / (double)m->view.voxelSize);  // as the raycast computes it
+    const Affine P = affine_of(T);
+    out->centre = make_float3(P.t0 * vsInv, P.t1 * vsInv, P.t2 * vsInv);
+    out->grow = ray_drift_margin(W, H, intrinsics_of(K), P.t0, P.t1, P.t2, vsInv);
+    return VM_OK;
 }
 
 // The rotation block of a column-major camToWorld is orthonormal: every
@@ -426,8 +446,13 @@ int vm_integrate(vm_map* m, const void* depth, int64_t pitch, int32_t W, int32_t
     m->visCountZeroed = false;
     DepthImage d{(const uint8_t*)depth, pitch, W, H, fmt, scale};
     ColorImage c{(const uint8_t*)rgb, rgb_pitch};
-    VM_HIP(launch_visible_fuse(m->view, d, c, intrinsics_of(K), affine_of(Ti), s), "visibility + integration");
+    BoundsAtPose bp;
+    bool withBounds = false;
+    if (int rc = fusion_bounds(m, W, H, K, T, s, &bp, &withBounds)) return rc;
+    VM_HIP(launch_visible_fuse(m->view, d, c, intrinsics_of(K), affine_of(Ti), s, withBounds ? &bp : nullptr),
+           "visibility + integration");
     remember_visibility(m, W, H, K, T);
+    if (withBounds) remember_bounds(m, W, H, K, T);
     if (out_visible) {
         int n = 0;
         if (int rc = read_counter(m, kVisibleCount, s, &n)) return rc;
@@ -453,8 +478,13 @@ int vm_fuse_depth(vm_map* m, const void* depth, int64_t pitch, int32_t W, int32_
     ColorImage c{(const uint8_t*)rgb, rgb_pitch};
     VM_HIP(launch_allocate(m->view, d, k, affine_of(T), s), "allocation");
     m->epoch++;
-    VM_HIP(launch_visible_fuse(m->view, d, c, k, affine_of(Ti), s), "visibility + integration");
+    BoundsAtPose bp;
+    bool withBounds = false;
+    if (int rc = fusion_bounds(m, W, H, K, T, s, &bp, &withBounds)) return rc;
+    VM_HIP(launch_visible_fuse(m->view, d, c, k, affine_of(Ti), s, withBounds ? &bp : nullptr),
+           "visibility + integration");
     remember_visibility(m, W, H, K, T);
+    if (withBounds) remember_bounds(m, W, H, K, T);
     if (out_visible) {
         int n = 0;
         if (int rc = read_counter(m, kVisibleCount, s, &n)) return rc;
@@ -490,12 +520,14 @@ int vm_raycast(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H, cons
     float Ti[16];
     if (!inverse4(T, Ti)) return fail(VM_ERR_INVALID_ARG, "camToWorld is singular");
     RayBounds rb;
-    if (int rc = ray_bounds_for(m, W, H, (hipStream_t)stream, &rb)) return rc;
     const bool rigid = rigid_rotation(T);
+    const bool ready = rigid && bounds_current(m, W, H, K, T);
+    if (ready) rb = RayBounds{m->rayKeys, m->rayKeysTiles, m->rbEpoch};
+    else if (int rc = ray_bounds_for(m, W, H, (hipStream_t)stream, &rb)) return rc;
     VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, intrinsics_of(K), affine_of(T), affine_of(Ti), false,
-                          (hipStream_t)stream, rigid ? &rb : nullptr),
+                          (hipStream_t)stream, rigid ? &rb : nullptr, ready),
            "raycast");
-    if (rigid) remember_bounds(m, W, H, K, T);
+    if (rigid && !ready) remember_bounds(m, W, H, K, T);
     return VM_OK;
 }
 
@@ -514,11 +546,14 @@ int vm_render_scene(vm_map* m, float* vmap, int64_t pitch, int32_t W, int32_t H,
         remember_visibility(m, W, H, K, T);
     }
     RayBounds rb;
-    if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
     const bool rigid = rigid_rotation(T);
-    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s, rigid ? &rb : nullptr),
+    const bool ready = rigid && bounds_current(m, W, H, K, T);
+    if (ready) rb = RayBounds{m->rayKeys, m->rayKeysTiles, m->rbEpoch};
+    else if (int rc = ray_bounds_for(m, W, H, s, &rb)) return rc;
+    VM_HIP(launch_raycast(m->view, vmap, pitch, W, H, k, affine_of(T), affine_of(Ti), true, s,
+                          rigid ? &rb : nullptr, ready),
            "raycast");
-    if (rigid) remember_bounds(m, W, H, K, T);
+    if (rigid && !ready) remember_bounds(m, W, H, K, T);
     return VM_OK;
 }
 

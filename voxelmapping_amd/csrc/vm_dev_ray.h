@@ -121,17 +121,15 @@ struct RayReader {
     // the block) instead of one cache lookup per corner. With HAVE0 the value
     // of corner 0 (the voxel floor(q) itself) is the caller's s0 / ok0, the
     // march's own sample of it.
-    template <bool HAVE0>
-    __device__ __forceinline__ float interp_in(float3 q, int X0, int Y0, int Z0, int bx, int by, int bz, int p,
-                                               float s0, bool ok0, bool& valid) {
-        const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
-        const int X1 = floor_i(q.x + 1.f), Y1 = floor_i(q.y + 1.f), Z1 = floor_i(q.z + 1.f);
+    // pb[c], c = i | j << 1 | l << 2: the pool offset of the block of the
+    // corners whose coordinates are X_i, Y_j, Z_l (i, j, l in {0, 1}), given
+    // the block (bx, by, bz) of (X0, Y0, Z0) at offset p and the corners
+    // X1, Y1, Z1. A combination is looked up only when every axis it sets
+    // crosses into the next block; otherwise it is the block of c with the
+    // bits of the non-crossing axes cleared.
+    __device__ __forceinline__ void corner_blocks(int X1, int Y1, int Z1, int bx, int by, int bz, int p, int pb[8]) {
         const int bx1 = vox_to_blk(X1), by1 = vox_to_blk(Y1), bz1 = vox_to_blk(Z1);
         const bool sx = bx1 != bx, sy = by1 != by, sz = bz1 != bz;
-        // pb[c], c = i | j << 1 | l << 2: the block of the corners whose
-        // coordinates are X_i, Y_j, Z_l (the same block as c with a bit
-        // cleared where that axis does not cross)
-        int pb[8];
         pb[0] = p;
         pb[1] = sx ? block(bx1, by, bz) : p;
         pb[2] = sy ? block(bx, by1, bz) : p;
@@ -146,6 +144,15 @@ struct RayReader {
                 : sx         ? pb[1]
                 : sy         ? pb[2]
                 : pb[4];
+    }
+
+    template <bool HAVE0>
+    __device__ __forceinline__ float interp_in(float3 q, int X0, int Y0, int Z0, int bx, int by, int bz, int p,
+                                               float s0, bool ok0, bool& valid) {
+        const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+        const int X1 = floor_i(q.x + 1.f), Y1 = floor_i(q.y + 1.f), Z1 = floor_i(q.z + 1.f);
+        int pb[8];
+        corner_blocks(X1, Y1, Z1, bx, by, bz, p, pb);
         const int lx0 = vox_to_loc(X0), lx1 = vox_to_loc(X1);
         const int ly0 = vox_to_loc(Y0), ly1 = vox_to_loc(Y1);
         const int lz0 = vox_to_loc(Z0), lz1 = vox_to_loc(Z1);
@@ -227,14 +234,19 @@ __device__ inline void trilinear(RayReader<FMT>& rd, float3 q, Trilinear& t) {
     const int X0 = floor_i(q.x), X1 = floor_i(q.x + 1.f);
     const int Y0 = floor_i(q.y), Y1 = floor_i(q.y + 1.f);
     const int Z0 = floor_i(q.z), Z1 = floor_i(q.z + 1.f);
-    t.idx[0] = rd.voxel(X0, Y0, Z0);
-    t.idx[1] = rd.voxel(X1, Y0, Z0);
-    t.idx[2] = rd.voxel(X0, Y1, Z0);
-    t.idx[3] = rd.voxel(X1, Y1, Z0);
-    t.idx[4] = rd.voxel(X0, Y0, Z1);
-    t.idx[5] = rd.voxel(X1, Y0, Z1);
-    t.idx[6] = rd.voxel(X0, Y1, Z1);
-    t.idx[7] = rd.voxel(X1, Y1, Z1);
+    {
+        const int bx = vox_to_blk(X0), by = vox_to_blk(Y0), bz = vox_to_blk(Z0);
+        int pb[8];
+        rd.corner_blocks(X1, Y1, Z1, bx, by, bz, rd.block(bx, by, bz), pb);
+        const int lx0 = vox_to_loc(X0), lx1 = vox_to_loc(X1);
+        const int ly0 = vox_to_loc(Y0), ly1 = vox_to_loc(Y1);
+        const int lz0 = vox_to_loc(Z0), lz1 = vox_to_loc(Z1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            t.idx[k] = pb[k] < 0 ? -1
+                                 : (int64_t)pb[k] + local_index((k & 1) ? lx1 : lx0, (k & 2) ? ly1 : ly0,
+                                                                (k & 4) ? lz1 : lz0);
+    }
     float s[8];
     t.valid = true;
 #pragma unroll
@@ -558,11 +570,13 @@ __device__ __forceinline__ void raycast_pixel(const MapView& m, float* out, int 
     r.distEnd = fminf(r.distEnd, farLim);
     ray_skip(r, nearLim);
     RayReader<FMT> rd(m);
-    // The loop takes, per pass, one box of a walk through empty blocks for
-    // the lanes that are walking and one march step for the others, so that
-    // the lanes of a wave walk side by side: a whole walk run at once by one
-    // lane is a loop of additions that its other lanes wait through, one
-    // lane after the other.
+    // The march loop (ray_tracing.cu:355-379) takes, per pass, one box of a
+    // walk through empty blocks for the lanes that are walking and one march
+    // step for the others, so that the lanes of a wave walk side by side: a
+    // whole walk run at once by one lane is a loop of additions that its
+    // other lanes wait through, one lane after the other. (Written out here
+    // and in the pose kernel: as a shared inline function it compiled to a
+    // raycast 12 % slower.)
     int st = kRayGoing;
     Walk wk{0, 0, 0, 0, false};
     do {

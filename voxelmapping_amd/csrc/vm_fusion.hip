@@ -1,7 +1,10 @@
 // Single-map kernels of reset, block allocation, visibility and TSDF
 // integration for gfx950 (device functions in vm_dev_fusion.h).
+#include <algorithm>
+
 #include "vm_common.h"
 #include "vm_dev_fusion.h"
+#include "vm_dev_ray.h"
 #include "vm_launch.h"
 
 namespace vmamd {
@@ -78,6 +81,22 @@ __global__ __launch_bounds__(256) void k_visible(MapView m, Intrinsics K, Affine
     const int A = allocated_count(m.ctr, m.nBlock);
     for (int chunk = blockIdx.x * 256; chunk < A; chunk += gridDim.x * 256)
         visible_chunk(m, K, Tinv, cols, rows, chunk, A, sh);
+}
+
+// The same, plus the per-tile ray-distance bounds of a raycast at this pose
+// (ray_bounds_chunk, vm_dev_ray.h), which the raycast that follows at the
+// same view then reuses: workgroups from the front of the grid take the
+// 256-block visibility chunks, workgroups from its back the 32-block bounds
+// chunks, so both run side by side in one launch.
+__global__ __launch_bounds__(256) void k_visible_bounds(MapView m, Intrinsics K, Affine Tinv, int cols, int rows,
+                                                        RayBounds rb, float3 c, float grow) {
+    __shared__ VisibleShared sh;
+    __shared__ BoundsShared<kBoundsChunk> bsh;
+    const int A = allocated_count(m.ctr, m.nBlock);
+    for (int chunk = blockIdx.x * 256; chunk < A; chunk += gridDim.x * 256)
+        visible_chunk(m, K, Tinv, cols, rows, chunk, A, sh);
+    for (int base = (gridDim.x - 1 - blockIdx.x) * kBoundsChunk; base < A; base += gridDim.x * kBoundsChunk)
+        ray_bounds_chunk<kBoundsChunk>(m, rb, base, A, cols, rows, K, Tinv, c, m.voxelSize, grow, bsh);
 }
 
 // Integration of the compacted visible table, one wave per visible block,
@@ -188,8 +207,15 @@ hipError_t launch_allocate(const MapView& m, const DepthImage& d, const Intrinsi
 }
 
 hipError_t launch_visible_compact(const MapView& m, int cols, int rows, const Intrinsics& K, const Affine& Tinv,
-                                  hipStream_t s) {
-    hipLaunchKernelGGL(k_visible, dim3(grid_for_chunks(m)), dim3(256), 0, s, m, K, Tinv, cols, rows);
+                                  hipStream_t s, const BoundsAtPose* bounds) {
+    if (bounds && bounds->rb.keys) {
+        const int need = (m.nBlock + kBoundsChunk - 1) / kBoundsChunk, gb = num_compute_units() * 8;
+        const int g = std::max(grid_for_chunks(m), std::min(need, gb));
+        hipLaunchKernelGGL(k_visible_bounds, dim3(g), dim3(256), 0, s, m, K, Tinv, cols, rows, bounds->rb,
+                           bounds->centre, bounds->grow);
+    } else {
+        hipLaunchKernelGGL(k_visible, dim3(grid_for_chunks(m)), dim3(256), 0, s, m, K, Tinv, cols, rows);
+    }
     return hipGetLastError();
 }
 
@@ -203,8 +229,8 @@ hipError_t launch_integrate(const MapView& m, const DepthImage& d, const ColorIm
 }
 
 hipError_t launch_visible_fuse(const MapView& m, const DepthImage& d, const ColorImage& c, const Intrinsics& K,
-                               const Affine& Tinv, hipStream_t s) {
-    hipError_t e = launch_visible_compact(m, d.width, d.height, K, Tinv, s);
+                               const Affine& Tinv, hipStream_t s, const BoundsAtPose* bounds) {
+    hipError_t e = launch_visible_compact(m, d.width, d.height, K, Tinv, s, bounds);
     if (e != hipSuccess) return e;
     return launch_integrate(m, d, c, K, Tinv, s);
 }

@@ -12,13 +12,24 @@ hipError_t launch_rebuild_index(const MapView& m, hipStream_t s);
 // CreateBlockLineTracingFunctor, depth_fusion.cu:8-60; also zeroes the visible count
 hipError_t launch_allocate(const MapView& m, const DepthImage& d, const Intrinsics& K,
                            const Affine& camToWorld, hipStream_t s);
-// CheckEntryVisibilityFunctor + DepthFusionFunctor/ImageFusionFunctor fused
+// The raycast's per-tile ray-distance bounds (RayBounds, vm_dev_ray.h) at a
+// pose, computed in the visibility launch of a fusion at that pose: the
+// camera centre and the boxes' growth (ray_drift_margin) in voxels.
+struct BoundsAtPose {
+    RayBounds rb;
+    float3 centre;
+    float grow;
+};
+// CheckEntryVisibilityFunctor + DepthFusionFunctor/ImageFusionFunctor fused;
+// with `bounds`, the visibility launch also computes the ray bounds of a
+// raycast at the same pose, image size and intrinsics
 hipError_t launch_visible_fuse(const MapView& m, const DepthImage& d, const ColorImage& c,
-                               const Intrinsics& K, const Affine& worldToCam, hipStream_t s);
+                               const Intrinsics& K, const Affine& worldToCam, hipStream_t s,
+                               const BoundsAtPose* bounds = nullptr);
 // the two halves of launch_visible_fuse: compaction of the visible blocks
 // (the count must be zero beforehand) and integration of the compacted list
 hipError_t launch_visible_compact(const MapView& m, int cols, int rows, const Intrinsics& K,
-                                  const Affine& worldToCam, hipStream_t s);
+                                  const Affine& worldToCam, hipStream_t s, const BoundsAtPose* bounds = nullptr);
 hipError_t launch_integrate(const MapView& m, const DepthImage& d, const ColorImage& c, const Intrinsics& K,
                             const Affine& worldToCam, hipStream_t s);
 // CheckEntryVisibilityFunctor alone (CheckVisibleBlocks, depth_fusion.cu:494-539)
@@ -30,7 +41,7 @@ hipError_t launch_visible(const MapView& m, int width, int height, const Intrins
 // for this image size and a fresh epoch, computed before the march
 hipError_t launch_raycast(const MapView& m, float* vmap, int64_t pitch, int width, int height,
                           const Intrinsics& K, const Affine& camToWorld, const Affine& worldToCam,
-                          bool gate, hipStream_t s, const RayBounds* bounds = nullptr);
+                          bool gate, hipStream_t s, const RayBounds* bounds = nullptr, bool boundsReady = false);
 // tiles of a width x height image (the bounds hold 2 (tiles + 1) 64-bit keys)
 inline int ray_bounds_tiles(int width, int height) { return ((width + 7) / 8) * ((height + 7) / 8); }
 // RenderingBlockDelegate + FillRenderingBlockFunctor, ray_tracing.cu:13-234

@@ -71,9 +71,14 @@ __device__ inline bool surface_depth(RayReader<FMT>& rd, int x, int y, const Int
     Ray r;
     ray_begin(r, x, y, K, T, vsInv);
     const float3 o = r.res;
-    int st;
-    while ((st = ray_step(r, rd, rtStep)) == kRayGoing) {
-    }
+    // the raycast's march loop: walks through empty blocks one box per pass
+    // (raycast_pixel, vm_dev_ray.h)
+    int st = kRayGoing;
+    Walk wk{0, 0, 0, 0, false};
+    do {
+        if (wk.on) wk.on = walk_box(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, wk.bx, wk.by, wk.bz, wk.steps);
+        if (!wk.on) st = ray_step(r, rd, rtStep, &wk);
+    } while (st == kRayGoing);
     if (st != kRaySurface) return false;
     float step = r.sdf * rtStep;
     float3 p = make_float3(r.res.x + step * r.dir.x, r.res.y + step * r.dir.y, r.res.z + step * r.dir.z);

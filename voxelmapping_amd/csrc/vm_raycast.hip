@@ -124,7 +124,8 @@ hipError_t launch_ray_bounds(const MapView& m, const RayBounds& rb, int W, int H
 }
 
 hipError_t launch_raycast(const MapView& m, float* vmap, int64_t pitch, int W, int H, const Intrinsics& K,
-                          const Affine& pose, const Affine& Tinv, bool gate, hipStream_t s, const RayBounds* bounds) {
+                          const Affine& pose, const Affine& Tinv, bool gate, hipStream_t s, const RayBounds* bounds,
+                          bool boundsReady) {
     const int tilesX = (W + 7) / 8, tilesY = (H + 7) / 8;
     const int nTiles = tilesX * tilesY;
     const int perXcd = (nTiles + 7) / 8;
@@ -136,8 +137,10 @@ hipError_t launch_raycast(const MapView& m, float* vmap, int64_t pitch, int W, i
     if (bounds) {
         rb = *bounds;
         margin = ray_drift_margin(W, H, K, pose.t0, pose.t1, pose.t2, vsInv);
-        hipError_t e = launch_ray_bounds(m, rb, W, H, K, pose, Tinv, margin, s);
-        if (e != hipSuccess) return e;
+        if (!boundsReady) {
+            hipError_t e = launch_ray_bounds(m, rb, W, H, K, pose, Tinv, margin, s);
+            if (e != hipSuccess) return e;
+        }
     }
     switch (m.fmt) {
         case kFmtRef:
