@@ -34,6 +34,9 @@ S_VOXEL_NO_COLOR = 8  # float TSDF + float weight when no colour is fused
 # timed steps whose every stage is bracketed by HIP events (each event is a
 # command on the stream; the raycast stage is bracketed on every step)
 STAGE_EVERY = 8
+# a spin kernel of this many GPU clock cycles (~10 ms) holds the stream while
+# the host queues a timed batch of short calls behind it
+HOLD_CYCLES = 20_000_000
 
 
 def stage_bytes(P, V):
@@ -340,6 +343,7 @@ def mesh_bench(m, reps: int = 5, cap: int = 20_000_000) -> dict:
     m.polygonize(verts, norms)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     torch.cuda.synchronize()
+    torch.cuda._sleep(HOLD_CYCLES)  # the calls are queued before the GPU reaches them
     ev[0].record()
     for _ in range(reps):
         _, tris = m.polygonize(verts, norms)
@@ -381,12 +385,14 @@ def adjoint_bench(reps: int = 20) -> dict:
     gp = torch.zeros(6, dtype=torch.float64, device="cuda")
     m.raycast(vm, K, T)
     st = m.raycast_backward(vm, K, T, gt, gp, grad_vertex=gv, grad_normal=gn)
-    # each pass queued reps times back to back, so the clock sees GPU time
+    # each pass queued reps times back to back behind a spin kernel that
+    # holds the GPU until all of them are queued, so the clock sees GPU time
     # rather than the launches' host latency
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     fwd = bwd = 0.0
     for _ in range(3):
         torch.cuda.synchronize()
+        torch.cuda._sleep(HOLD_CYCLES)
         ev[0].record()
         for _ in range(reps):
             m.raycast(vm, K, T)

@@ -89,6 +89,90 @@ __device__ inline float3 reverse_sample(RayReader<kFmtF32>& rd, float3 q, bool i
     return make_float3(0.f, 0.f, 0.f);
 }
 
+// Adjoint of computeNormalKernel gathered per vertex, for a T x T tile of
+// vertices at (X0, Y0): g_v[q] = g_vertex[q] + the contributions of every
+// normal pixel whose stencil reads q. Normals computed from a missed vertex
+// (x = NaN) pass no gradient. The tile's vertices are reached by the normal
+// pixels of the tile grown by one; each of those (T + 2)^2 normal pixels'
+// two stencil-difference gradients (ga for v(x+1) - v(x-1), gb for
+// v(y+1) - v(y-1)) is computed once into LDS by normals_adjoint_halo, and
+// each vertex then gathers what its neighbours' stencils give it
+// (vertex_grad).
+template <int T>
+struct NormalsAdjointShared {
+    float3 ga[(T + 2) * (T + 2)], gb[(T + 2) * (T + 2)];
+    unsigned char on[(T + 2) * (T + 2)];
+};
+
+__device__ inline float3 vmap_at(const float* vmap, int64_t vpitch, int yy, int xx) {
+    const float4 v = *((const float4*)((const uint8_t*)vmap + (int64_t)yy * vpitch) + xx);
+    return make_float3(v.x, v.y, v.z);
+}
+
+// the halo's stencil gradients, by `nthreads` threads (index tid); the caller
+// synchronizes before vertex_grad
+template <int T>
+__device__ inline void normals_adjoint_halo(const float* vmap, int64_t vpitch, const float* gn, int64_t gnpitch, int W,
+                                            int H, int X0, int Y0, int tid, int nthreads,
+                                            NormalsAdjointShared<T>& sh) {
+    constexpr int NH = T + 2;
+    for (int t = tid; t < NH * NH; t += nthreads) {
+        const int px = X0 - 1 + t % NH, py = Y0 - 1 + t / NH;
+        bool on = false;
+        if (gn && px >= 0 && py >= 0 && px < W - 1 && py < H - 1) {  // the pixels computeNormal writes
+            const float4 g4 = *((const float4*)((const uint8_t*)gn + (int64_t)py * gnpitch) + px);
+            const float3 g = make_float3(g4.x, g4.y, g4.z);
+            if (g.x != 0.f || g.y != 0.f || g.z != 0.f) {
+                const int x10 = max(px - 1, 0), x01 = min(px + 1, W), y10 = max(py - 1, 0), y01 = min(py + 1, H);
+                const float3 v00 = vmap_at(vmap, vpitch, py, x10), v01 = vmap_at(vmap, vpitch, py, x01);
+                const float3 v10 = vmap_at(vmap, vpitch, y10, px), v11 = vmap_at(vmap, vpitch, y01, px);
+                if (!(isnan(v00.x) || isnan(v01.x) || isnan(v10.x) || isnan(v11.x))) {
+                    const float3 a = sub3(v01, v00), b = sub3(v11, v10);
+                    const float3 cr = crs3(a, b);
+                    const float n2 = sum3(cr.x * cr.x, cr.y * cr.y, cr.z * cr.z);
+                    float3 gc = g;
+                    if (n2 > 0.f) {
+                        const float nn = sqrtf(n2);
+                        const float3 nrm = mul3(cr, 1.0f / nn);
+                        gc = mul3(sub3(g, mul3(nrm, dot3f(nrm, g))), 1.0f / nn);
+                    }
+                    sh.ga[t] = crs3(b, gc);
+                    sh.gb[t] = crs3(gc, a);
+                    on = true;
+                }
+            }
+        }
+        sh.on[t] = on;
+    }
+}
+
+// total vertex gradient of vertex (x, y) of the tile
+template <int T>
+__device__ inline float3 vertex_grad(const float* gvin, int64_t gvpitch, int W, int H, int X0, int Y0, int x, int y,
+                                     const NormalsAdjointShared<T>& sh) {
+    constexpr int NH = T + 2;
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    if (gvin) {
+        const float4 g = *((const float4*)((const uint8_t*)gvin + (int64_t)y * gvpitch) + x);
+        acc = make_float3(g.x, g.y, g.z);
+    }
+    // normal pixels (px, py) whose stencil can read (x, y)
+    const int cand[5][2] = {{x, y}, {x - 1, y}, {x + 1, y}, {x, y - 1}, {x, y + 1}};
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const int px = cand[c][0], py = cand[c][1];
+        const int t = (py - Y0 + 1) * NH + (px - X0 + 1);
+        if (!sh.on[t]) continue;  // off the written pixels, no upstream gradient, or a missed vertex
+        const int x10 = max(px - 1, 0), x01 = min(px + 1, W), y10 = max(py - 1, 0), y01 = min(py + 1, H);
+        const float3 ga = sh.ga[t], gb = sh.gb[t];
+        if (py == y && x01 == x) acc = add3(acc, ga);
+        if (py == y && x10 == x) acc = sub3(acc, ga);
+        if (px == x && y01 == y) acc = add3(acc, gb);
+        if (px == x && y10 == y) acc = sub3(acc, gb);
+    }
+    return acc;
+}
+
 struct StackRec {
     float x, y, z;
     int meta;  // bit 0: interp sample; bits 1..: unit steps after this step
@@ -96,7 +180,9 @@ struct StackRec {
 
 // One wave per 8x8-pixel tile, tiles in the raycast's XCD-banded order; each
 // wave writes its 6 pose parts.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_raycast_backward(MapView m, const float* gv, int64_t gpitch, int W, int H,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_raycast_backward(MapView m, const float* vmap, int64_t vpitch,
+                                                         const float* gVertex, int64_t gvpitch, const float* gNormal,
+                                                         int64_t gnpitch, int W, int H,
                                                          Intrinsics K, Affine pose, Affine Tinv, float vsInv,
                                                          float rtStep, float* gTsdf, float* poseParts,
                                                          StackRec* stack, int* stats, int tilesX, int tilesPerXcd,
@@ -112,13 +198,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     GradAcc acc{s_keys, s_vals, gTsdf};
     const int g = blockIdx.x;
     const int tile = (g & 7) * tilesPerXcd + (g >> 3);
-    const int x = (tile % tilesX) * 8 + (lane & 7);
-    const int y = (tile / tilesX) * 8 + (lane >> 3);
+    const int X0 = (tile % tilesX) * 8, Y0 = (tile / tilesX) * 8;
+    const int x = X0 + (lane & 7);
+    const int y = Y0 + (lane >> 3);
+    // the vertex gradients of the tile: the upstream vertex gradient plus the
+    // normals' adjoint (computeNormal's stencil over the tile grown by one)
+    __shared__ NormalsAdjointShared<8> nsh;
+    if (tile < nTiles) normals_adjoint_halo<8>(vmap, vpitch, gNormal, gnpitch, W, H, X0, Y0, lane, 64, nsh);
+    __syncthreads();
     float part[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool overflow = false, hit = false;
     if (tile < nTiles && x < W && y < H) {
-        const float4 g4 = *((const float4*)((const uint8_t*)gv + (int64_t)y * gpitch) + x);
-        const float3 gvec = make_float3(g4.x, g4.y, g4.z);
+        const float3 gvec = vertex_grad<8>(gVertex, gvpitch, W, H, X0, Y0, x, y, nsh);
         if (gvec.x != 0.f || gvec.y != 0.f || gvec.z != 0.f) {
             const int64_t P = (int64_t)W * H, pix = (int64_t)y * W + x;
             // ---- forward recompute (same decisions as raycast_pixel) ----
@@ -323,82 +414,9 @@ __global__ __launch_bounds__(kReduceThreads) void k_pose_reduce(const float* par
     if (threadIdx.x == 0) gPose[k] += s[0];
 }
 
-// Adjoint of computeNormalKernel gathered per vertex: g_v[q] = g_vertex[q] +
-// the contributions of every normal pixel whose stencil reads q. Normals
-// computed from a missed vertex (x = NaN) pass no gradient. A 16x16 tile of
-// vertices is reached by the normal pixels of the tile grown by one; each of
-// those 18x18 normal pixels' two stencil-difference gradients (ga for
-// v(x+1) - v(x-1), gb for v(y+1) - v(y-1)) is computed once into LDS, and each
-// vertex then gathers what its neighbours' stencils give it.
-constexpr int kNbT = 16, kNbH = kNbT + 2;
-__global__ __launch_bounds__(256) void k_normals_backward(const float* vmap, int64_t vpitch, const float* gvin,
-                                                          int64_t gvpitch, const float* gn, int64_t gnpitch,
-                                                          float* gout, int64_t gopitch, int W, int H) {
-    __shared__ float3 s_ga[kNbH * kNbH], s_gb[kNbH * kNbH];
-    __shared__ unsigned char s_on[kNbH * kNbH];
-    const int X0 = blockIdx.x * kNbT, Y0 = blockIdx.y * kNbT;
-    auto V = [&](int yy, int xx) {
-        const float4 v = *((const float4*)((const uint8_t*)vmap + (int64_t)yy * vpitch) + xx);
-        return make_float3(v.x, v.y, v.z);
-    };
-    if (gn) {
-        for (int t = threadIdx.x; t < kNbH * kNbH; t += 256) {
-            const int px = X0 - 1 + t % kNbH, py = Y0 - 1 + t / kNbH;
-            bool on = false;
-            if (px >= 0 && py >= 0 && px < W - 1 && py < H - 1) {  // the pixels computeNormal writes
-                const float4 g4 = *((const float4*)((const uint8_t*)gn + (int64_t)py * gnpitch) + px);
-                const float3 g = make_float3(g4.x, g4.y, g4.z);
-                if (g.x != 0.f || g.y != 0.f || g.z != 0.f) {
-                    const int x10 = max(px - 1, 0), x01 = min(px + 1, W), y10 = max(py - 1, 0), y01 = min(py + 1, H);
-                    const float3 v00 = V(py, x10), v01 = V(py, x01), v10 = V(y10, px), v11 = V(y01, px);
-                    if (!(isnan(v00.x) || isnan(v01.x) || isnan(v10.x) || isnan(v11.x))) {
-                        const float3 a = sub3(v01, v00), b = sub3(v11, v10);
-                        const float3 cr = crs3(a, b);
-                        const float n2 = sum3(cr.x * cr.x, cr.y * cr.y, cr.z * cr.z);
-                        float3 gc = g;
-                        if (n2 > 0.f) {
-                            const float nn = sqrtf(n2);
-                            const float3 nrm = mul3(cr, 1.0f / nn);
-                            gc = mul3(sub3(g, mul3(nrm, dot3f(nrm, g))), 1.0f / nn);
-                        }
-                        s_ga[t] = crs3(b, gc);
-                        s_gb[t] = crs3(gc, a);
-                        on = true;
-                    }
-                }
-            }
-            s_on[t] = on;
-        }
-    }
-    __syncthreads();
-    const int x = X0 + (threadIdx.x & 15), y = Y0 + (threadIdx.x >> 4);
-    if (x >= W || y >= H) return;
-    float3 acc = make_float3(0.f, 0.f, 0.f);
-    if (gvin) {
-        const float4 g = *((const float4*)((const uint8_t*)gvin + (int64_t)y * gvpitch) + x);
-        acc = make_float3(g.x, g.y, g.z);
-    }
-    if (gn) {
-        // normal pixels (px, py) whose stencil can read (x, y)
-        const int cand[5][2] = {{x, y}, {x - 1, y}, {x + 1, y}, {x, y - 1}, {x, y + 1}};
-        for (int c = 0; c < 5; ++c) {
-            const int px = cand[c][0], py = cand[c][1];
-            const int t = (py - Y0 + 1) * kNbH + (px - X0 + 1);
-            if (!s_on[t]) continue;  // off the written pixels, no upstream gradient, or a missed vertex
-            const int x10 = max(px - 1, 0), x01 = min(px + 1, W), y10 = max(py - 1, 0), y01 = min(py + 1, H);
-            const float3 ga = s_ga[t], gb = s_gb[t];
-            if (py == y && x01 == x) acc = add3(acc, ga);
-            if (py == y && x10 == x) acc = sub3(acc, ga);
-            if (px == x && y01 == y) acc = add3(acc, gb);
-            if (px == x && y10 == y) acc = sub3(acc, gb);
-        }
-    }
-    *((float4*)((uint8_t*)gout + (int64_t)y * gopitch) + x) = make_float4(acc.x, acc.y, acc.z, 0.f);
-}
-
 }  // namespace
 
-// scratch layout: [stack | pose parts (256-byte aligned) | vertex gradients]
+// scratch layout: [stack | pose parts (256-byte aligned)]
 // one launch slot per 8x8 tile, rounded up to whole XCD bands
 static size_t backward_waves(int W, int H) {
     const size_t tiles = (size_t)((W + 7) / 8) * ((H + 7) / 8);
@@ -407,7 +425,7 @@ static size_t backward_waves(int W, int H) {
 static size_t parts_bytes(int W, int H) { return ((sizeof(float) * 6 * backward_waves(W, H) + 255) / 256) * 256; }
 
 size_t raycast_backward_scratch_bytes(int W, int H) {
-    return sizeof(StackRec) * kStackDepth * (size_t)W * H + parts_bytes(W, H) + sizeof(float) * 4 * (size_t)W * H;
+    return sizeof(StackRec) * kStackDepth * (size_t)W * H + parts_bytes(W, H);
 }
 
 hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t vpitch, const float* gVertex,
@@ -421,11 +439,6 @@ hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t 
     StackRec* stack = (StackRec*)p;
     p += sizeof(StackRec) * kStackDepth * (size_t)W * H;
     float* parts = (float*)p;
-    p += parts_bytes(W, H);
-    float* gv = (float*)p;
-    const int64_t gpitch = (int64_t)W * 16;
-    hipLaunchKernelGGL(k_normals_backward, dim3((W + kNbT - 1) / kNbT, (H + kNbT - 1) / kNbT), dim3(256), 0, s, vmap,
-                       vpitch, gVertex, gvpitch, gNormal, gnpitch, gv, gpitch, W, H);
     const float vsInv = (float)(1.0 / (double)m.voxelSize);
     const float rtStep = m.trunc / m.voxelSize;
     const float margin = ray_drift_margin(W, H, K, pose.t0, pose.t1, pose.t2, vsInv);
@@ -433,8 +446,9 @@ hipError_t launch_raycast_backward(const MapView& m, const float* vmap, int64_t 
         hipError_t e = launch_ray_bounds(m, rb, W, H, K, pose, Tinv, margin, s);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_raycast_backward, dim3((unsigned)waves), dim3(64), 0, s, m, gv, gpitch, W, H, K, pose, Tinv,
-                       vsInv, rtStep, gTsdf, parts, stack, stats, tilesX, waves / 8, nTiles, rb, margin);
+    hipLaunchKernelGGL(k_raycast_backward, dim3((unsigned)waves), dim3(64), 0, s, m, vmap, vpitch, gVertex, gvpitch,
+                       gNormal, gnpitch, W, H, K, pose, Tinv, vsInv, rtStep, gTsdf, parts, stack, stats, tilesX,
+                       waves / 8, nTiles, rb, margin);
     hipLaunchKernelGGL(k_pose_reduce, dim3(6), dim3(kReduceThreads), 0, s, parts, waves, gPose);
     return hipGetLastError();
 }
