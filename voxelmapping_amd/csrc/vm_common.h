@@ -207,6 +207,10 @@ template <> struct Vox<kFmtRef> {
         sdf = sdf_of((int16_t)(w & 0xffffu));
         valid = ((w >> 16) & 0xffu) != 0;
     }
+    // voxel `local` of the block at pool offset ptr (a multiple of 512)
+    __device__ static inline void read_local(const uint8_t* base, int ptr, int local, float& sdf, bool& valid) {
+        read(base, (int64_t)ptr + local, sdf, valid);
+    }
 };
 // vmap::VoxelRGB: bytes 0-1 sdf, 2 weight, 3-5 rgb (6 B, 2-byte aligned)
 template <> struct Vox<kFmtRefRGB> {
@@ -216,6 +220,9 @@ template <> struct Vox<kFmtRefRGB> {
         const uint16_t* p = (const uint16_t*)(base + i * 6);
         sdf = (float)(int16_t)p[0] / 32767.0f;
         valid = (p[1] & 0xffu) != 0;
+    }
+    __device__ static inline void read_local(const uint8_t* base, int ptr, int local, float& sdf, bool& valid) {
+        read(base, (int64_t)ptr + local, sdf, valid);
     }
 };
 // float TSDF + float weight + rgb8, 12 B per voxel, stored per block as two
@@ -231,6 +238,14 @@ template <> struct Vox<kFmtF32> {
     __host__ __device__ static inline int64_t field_offset(int64_t i) { return (i >> 9) * kBlockBytes + (i & 511) * 8; }
     __device__ static inline void read(const uint8_t* base, int64_t i, float& sdf, bool& valid) {
         const float2 v = *(const float2*)(base + field_offset(i));
+        sdf = v.x;
+        valid = v.y != 0.f;
+    }
+    // voxel `local` of the block at pool offset ptr (a multiple of 512): the
+    // block's byte offset from its 32-bit id, the voxel's within the block
+    __device__ static inline void read_local(const uint8_t* base, int ptr, int local, float& sdf, bool& valid) {
+        const float2 v = *(const float2*)(base + (uint64_t)(uint32_t)(ptr >> 9) * (uint64_t)kBlockBytes +
+                                          (uint32_t)(local << 3));
         sdf = v.x;
         valid = v.y != 0.f;
     }

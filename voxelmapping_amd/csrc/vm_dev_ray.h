@@ -168,10 +168,9 @@ struct RayReader {
             } else {
                 const int b = pb[k];
                 if (b >= 0)
-                    Vox<FMT>::read(m.voxels,
-                                   (int64_t)b + local_index((k & 1) ? lx1 : lx0, (k & 2) ? ly1 : ly0,
-                                                            (k & 4) ? lz1 : lz0),
-                                   v, ok);
+                    Vox<FMT>::read_local(m.voxels, b,
+                                         local_index((k & 1) ? lx1 : lx0, (k & 2) ? ly1 : ly0, (k & 4) ? lz1 : lz0),
+                                         v, ok);
             }
             s[k] = ok ? v : 1.0f;
             all = all && ok;
@@ -203,7 +202,11 @@ struct RayReader {
     // is within [-0.5, 0.5] (ray_tracing.cu:358-362).
     __device__ inline float sample(float3 q, int vx, int vy, int vz, int p, bool& valid) {
         const int lx = vox_to_loc(vx), ly = vox_to_loc(vy), lz = vox_to_loc(vz);
-        float sdf = read_at(p < 0 ? -1 : (int64_t)p + local_index(lx, ly, lz), valid);
+        float sdf = 1.0f;
+        bool v = false;
+        if (p >= 0) Vox<FMT>::read_local(m.voxels, p, local_index(lx, ly, lz), sdf, v);
+        valid = v;
+        if (!v) sdf = 1.0f;
         if (sdf <= 0.5f && sdf >= -0.5f)
             sdf = interp_in<true>(q, vx, vy, vz, vox_to_blk(vx), vox_to_blk(vy), vox_to_blk(vz), p, sdf, valid,
                                   valid);
@@ -442,7 +445,7 @@ __device__ inline int run_samples(const RayReader<FMT>& rd, float3& res, float& 
     for (int k = 0; k < kRunSpec; ++k) {
         sv[k] = 1.0f;
         ok[k] = false;
-        if (k < n) Vox<FMT>::read(rd.m.voxels, (int64_t)p + loc[k], sv[k], ok[k]);
+        if (k < n) Vox<FMT>::read_local(rd.m.voxels, p, loc[k], sv[k], ok[k]);
     }
     int acc = 0;
 #pragma unroll
