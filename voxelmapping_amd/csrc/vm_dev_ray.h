@@ -719,7 +719,12 @@ __device__ __forceinline__ void ray_bounds_blocks(const int4 b, const RayBounds&
                 const float3 p = xform(Tinv, w);
                 zmin = fminf(zmin, p.z);
                 zmax = fmaxf(zmax, p.z);
-                const float u = K.fx * p.x / p.z + K.cx, v = K.fy * p.y / p.z + K.cy;
+                // hardware reciprocal (1 ulp): u and v move by a few 2^-23
+                // of their size, far below the one pixel the rectangle is
+                // grown by wherever they fall near the image (farther out the
+                // rectangle is clipped to the image anyway)
+                const float iz = __builtin_amdgcn_rcpf(p.z);
+                const float u = K.fx * p.x * iz + K.cx, v = K.fy * p.y * iz + K.cy;
                 umin = fminf(umin, u); umax = fmaxf(umax, u);
                 vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
             }
