@@ -307,7 +307,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 // block, read together: unit steps, sdf stays 1
                 if (!valid && p >= 0) units += run_samples(rd, res, dist_s, dir, distEnd, p, bx, by, bz, 1.0f, false);
             }
-            if (depth > 0) stack[(int64_t)(depth - 1) * P + pix] = pend;
+            // the last record stays in registers (pend): the reverse sweep
+            // starts from it
             if (sdf <= 0.0f && !overflow) {
                 const float3 rk = res;
                 const bool kInterp = interpUsed;
@@ -341,8 +342,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                     lam = add3(lam, reverse_sample(rd, rk, kInterp, lam, dir, rtStep, acc));
                     // unit steps before the break, then the active steps in reverse
                     gDir = add3(gDir, mul3(lam, (float)units));
+                    // each record's load is issued one record ahead, so that it
+                    // is not a round trip of its own in the sweep
+                    StackRec next = pend;
                     for (int d = depth - 1; d >= 0; --d) {
-                        const StackRec rec = stack[(int64_t)d * P + pix];
+                        const StackRec rec = next;
+                        if (d > 0) next = stack[(int64_t)(d - 1) * P + pix];
                         const float3 q = make_float3(rec.x, rec.y, rec.z);
                         // the sample is read once: its value for d/d dir, its
                         // weights and gradient for dTSDF and lambda
