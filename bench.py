@@ -521,9 +521,10 @@ def main():
     if l2_hit is not None:
         roofline["l2_hit_rate"] = l2_hit
     if dom == "raycast":
-        # the raycast stage is two launches: the per-tile ray-distance bounds,
-        # then the march; the events bracket both
-        roofline["stage_kernels"] = ["k_ray_bounds", "k_raycast<2>"]
+        # the raycast stage is one launch, the march: its per-tile
+        # ray-distance bounds were computed by the fusion's visibility launch
+        # at the same pose (k_visible_bounds, in the integrate stage)
+        roofline["stage_kernels"] = ["k_raycast<2>"]
     out = {
         "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1000.0 * wall / args.steps, 4), "higher_is_better": True,
