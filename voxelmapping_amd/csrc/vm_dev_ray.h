@@ -322,13 +322,17 @@ __device__ __forceinline__ bool walk_box(const RayReader<FMT>& rd, float3& res, 
         q = kz >= q ? q : kz;
         int n = q >= 1.0f ? (int)q : 0;
         steps += n + 1;
-        for (; n >= 2; n -= 2) {
+        for (; n >= 4; n -= 4) {
+            res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+            dist_s += 1.0f;
+            res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+            dist_s += 1.0f;
             res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
             dist_s += 1.0f;
             res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
             dist_s += 1.0f;
         }
-        if (n) {
+        for (; n > 0; --n) {
             res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
             dist_s += 1.0f;
         }
@@ -541,13 +545,14 @@ __device__ inline bool ray_surface(Ray& r, RayReader<FMT>& rd, const Affine& Tin
 __device__ inline void ray_skip(Ray& r, float nearLim) {
     const float q = fminf(nearLim, r.distEnd - 2.0f) - r.dist_s;
     int n = q >= 1.0f ? (int)q : 0;
-    for (; n >= 2; n -= 2) {
-        r.res = make_float3(r.res.x + r.dir.x, r.res.y + r.dir.y, r.res.z + r.dir.z);
-        r.dist_s += 1.0f;
-        r.res = make_float3(r.res.x + r.dir.x, r.res.y + r.dir.y, r.res.z + r.dir.z);
-        r.dist_s += 1.0f;
+    for (; n >= 8; n -= 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            r.res = make_float3(r.res.x + r.dir.x, r.res.y + r.dir.y, r.res.z + r.dir.z);
+            r.dist_s += 1.0f;
+        }
     }
-    if (n) {
+    for (; n > 0; --n) {
         r.res = make_float3(r.res.x + r.dir.x, r.res.y + r.dir.y, r.res.z + r.dir.z);
         r.dist_s += 1.0f;
     }
