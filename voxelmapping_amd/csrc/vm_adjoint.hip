@@ -250,10 +250,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             {
                 // unit steps below the near bound (ray_skip), counted as units
                 const float q = fminf(nearLim, distEnd - 2.0f) - dist_s;
-                for (int n = q >= 1.0f ? (int)q : 0; n > 0; --n) {
+                int n = q >= 1.0f ? (int)q : 0;
+                units += n;
+                for (; n >= 8; n -= 8) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
+                        dist_s += 1.0f;
+                    }
+                }
+                for (; n > 0; --n) {
                     res = make_float3(res.x + dir.x, res.y + dir.y, res.z + dir.z);
                     dist_s += 1.0f;
-                    ++units;
                 }
             }
             // walks through empty blocks one box per pass, side by side with
@@ -273,13 +281,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                     valid = false;
                     continue;
                 }
-                sdf = rd.read_at(p < 0 ? -1 : (int64_t)p + local_index(vox_to_loc(vx), vox_to_loc(vy), vox_to_loc(vz)),
-                                 valid);
-                interpUsed = false;
-                if (sdf <= 0.5f && sdf >= -0.5f) {
-                    sdf = rd.interp(res, valid);
-                    interpUsed = true;
-                }
+                // the forward's sample (RayReader::sample), with its
+                // trilinear read taking the sample as its first corner
+                sdf = rd.sample(res, vx, vy, vz, p, valid);
+                interpUsed = rd.lastInterp;
                 if (sdf <= 0.0f) break;
                 if (sdf >= 0.f && last < 0.f) break;  // the reference returns here: no hit
                 step = valid ? fmaxf(sdf * rtStep, 1.0f) : 1.0f;

@@ -54,10 +54,11 @@ struct RayReader {
     int px, py, pz, pp;  // the one before
     int qx, qy, qz;      // brick (block >> 2) of `word`; INT_MIN: none yet
     unsigned long long word;
+    bool lastInterp;     // the last sample() took a trilinear read
 
     __device__ explicit RayReader(const MapView& mv)
         : m(mv), cx(INT_MIN), cy(INT_MIN), cz(INT_MIN), cp(-1), px(INT_MIN), py(INT_MIN), pz(INT_MIN), pp(-1),
-          qx(INT_MIN), qy(INT_MIN), qz(INT_MIN), word(0) {}
+          qx(INT_MIN), qy(INT_MIN), qz(INT_MIN), word(0), lastInterp(false) {}
 
     // false: no hash entry holds block (bx, by, bz)
     __device__ inline bool maybe_present(int bx, int by, int bz) {
@@ -207,7 +208,8 @@ struct RayReader {
         if (p >= 0) Vox<FMT>::read_local(m.voxels, p, local_index(lx, ly, lz), sdf, v);
         valid = v;
         if (!v) sdf = 1.0f;
-        if (sdf <= 0.5f && sdf >= -0.5f)
+        lastInterp = sdf <= 0.5f && sdf >= -0.5f;
+        if (lastInterp)
             sdf = interp_in<true>(q, vx, vy, vz, vox_to_blk(vx), vox_to_blk(vy), vox_to_blk(vz), p, sdf, valid,
                                   valid);
         return sdf;
