@@ -278,3 +278,43 @@ def test_raycast_non_rigid_pose(fused, scale):
     ov = o.raycast(w.width, w.height, w.K, Tc, init=0.0)
     assert _compare_vmaps(_gpu_raycast(g, w, Tc).cpu().numpy(), ov) > 0.5
     assert _compare_vmaps(_gpu_raycast(g, w, Tc, facade=True).cpu().numpy(), ov) > 0.5
+
+
+def test_render_reuses_fusion_bounds_only_at_its_view(oracle_threads):
+    """A fusion computes the raycast's per-tile ray bounds at its own pose in
+    its visibility launch; a render at that pose (render_scene and raycast)
+    reuses them, a render at another pose or after an allocation computes
+    its own. Every render equals the oracle's."""
+    import torch
+    from voxelmapping_amd import synth
+    w = synth.room_small(n_frames=8)
+    g, o = make_pair(w, fmt=2)
+    fuse_both(w, g, o, range(6), rgb=True)
+
+    def both(T, facade=True):
+        vm = _gpu_raycast(g, w, T, facade=facade).cpu().numpy()
+        ov = np.zeros_like(vm)
+        o.render_scene(ov, w.K, T)
+        return _compare_vmaps(vm, ov)
+
+    T5 = w.poses(6)[5]
+    assert both(T5) > 0.9  # the fusion's bounds
+    Tm = T5.reshape(4, 4).T.copy()
+    Tm[:3, 3] += np.array([0.001, 0.0, -0.001], np.float32)
+    assert both(Tm.T.reshape(-1)) > 0.9  # another pose: bounds of its own
+    assert both(T5, facade=False) > 0.9  # back at the fusion's pose
+    # an allocation alone adds empty blocks, which render as absent ones:
+    # the render after it equals the oracle's map without them
+    from parity import gpu_frame
+    d, c, dt, ct = gpu_frame(w, 7)
+    T7 = w.poses(8)[7]
+    g.allocate(dt, w.K, T7, depth_scale=w.depth_scale)
+    assert both(T5) > 0.9
+    # the integration of that frame computes the bounds at its pose, which
+    # the renders at that pose reuse
+    g.integrate(dt, w.K, T7, rgb=ct, depth_scale=w.depth_scale)
+    o.fuse(w.depth_metres(d), w.K, T7, rgb=c)
+    torch.cuda.synchronize()
+    assert both(T7) > 0.9
+    assert both(T7, facade=False) > 0.9
+    compare_maps(g, o, 2)
