@@ -103,14 +103,6 @@ struct RayReader {
         return (int64_t)p + local_index(vox_to_loc(vx), vox_to_loc(vy), vox_to_loc(vz));
     }
 
-    // read_sdf, ray_tracing.cu:272-286, for the voxel floor(q)
-    __device__ inline float read_at(int64_t i, bool& valid) {
-        float s = 1.0f;
-        bool v = false;
-        if (i >= 0) Vox<FMT>::read(m.voxels, i, s, v);
-        valid = v;
-        return v ? s : 1.0f;
-    }
 
     // read_sdf_interped, ray_tracing.cu:288-321, at q with floor(q) =
     // (X0, Y0, Z0) in block (bx, by, bz) at pool offset p (-1: absent).
@@ -199,8 +191,10 @@ struct RayReader {
     }
 
     // The march's sample at q, floor(q) = (vx, vy, vz) in block p (ptr or
-    // -1): read_sdf of floor(q), and read_sdf_interped of q when that value
-    // is within [-0.5, 0.5] (ray_tracing.cu:358-362).
+    // -1): read_sdf of floor(q) (ray_tracing.cu:272-286: the voxel's value
+    // when its weight is nonzero, else (1.0, invalid)), and
+    // read_sdf_interped of q when that value is within [-0.5, 0.5]
+    // (ray_tracing.cu:358-362).
     __device__ inline float sample(float3 q, int vx, int vy, int vz, int p, bool& valid) {
         const int lx = vox_to_loc(vx), ly = vox_to_loc(vy), lz = vox_to_loc(vz);
         float sdf = 1.0f;
@@ -349,17 +343,6 @@ __device__ __forceinline__ bool walk_box(const RayReader<FMT>& rd, float3& res, 
     return exact_range(cbx) && exact_range(cby) && exact_range(cbz) && rd.known_absent(cbx, cby, cbz);
 }
 
-// The whole walk from block (bx, by, bz): boxes until the march must take a
-// sample itself, or distEnd. Returns the number of unit steps taken.
-template <int FMT>
-__device__ inline int empty_walk(RayReader<FMT>& rd, float3& res, const float3 dir, const float3 inv, float& dist_s,
-                                 const float distEnd, const int bx, const int by, const int bz) {
-    int cbx = bx, cby = by, cbz = bz, steps = 0;
-    while (walk_box(rd, res, dir, inv, dist_s, distEnd, cbx, cby, cbz, steps)) {
-    }
-    return steps;
-}
-
 // State of one ray of MapRenderingDelegate::operator() (ray_tracing.cu:330-401)
 // between samples, so that a lane can run a ray in pieces.
 struct Ray {
@@ -483,25 +466,21 @@ struct Walk {
 // rtStep = trunc / voxelSize (ray_tracing.cu:431). RUNS (RunKinds): the
 // runs of samples taken after a sample that starts one (ray_run) -- for the
 // forward raycast only, as a run's samples are not visited one by one.
-// With a Walk, a walk through empty blocks is only started (wk->on, its
-// first block, the sample state it leaves: (1.0, invalid)) and the caller
-// takes its boxes; without one it is run here to its end.
+// A walk through empty blocks is only started here (wk.on, its first block,
+// the sample state it leaves: (1.0, invalid)); the caller takes its boxes
+// (walk_box), one per pass of its loop.
 template <int FMT, int RUNS = kRunsNone>
-__device__ __forceinline__ int ray_step(Ray& r, RayReader<FMT>& rd, float rtStep, Walk* wk = nullptr) {
+__device__ __forceinline__ int ray_step(Ray& r, RayReader<FMT>& rd, float rtStep, Walk& wk) {
     if (!(r.dist_s < r.distEnd)) return kRayMiss;  // loop condition; sdf > 0 or NaN here
     const float last = r.sdf;
     const int vx = floor_i(r.res.x), vy = floor_i(r.res.y), vz = floor_i(r.res.z);
     const int bx = vox_to_blk(vx), by = vox_to_blk(vy), bz = vox_to_blk(vz);
     const int p = rd.block(bx, by, bz);
     if (p < 0 && exact_range(bx) && exact_range(by) && exact_range(bz)) {
-        if (wk) {
-            wk->on = true;
-            wk->bx = bx;
-            wk->by = by;
-            wk->bz = bz;
-        } else {
-            empty_walk(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, bx, by, bz);
-        }
+        wk.on = true;
+        wk.bx = bx;
+        wk.by = by;
+        wk.bz = bz;
         r.sdf = 1.0f;
         r.valid = false;
         return kRayGoing;
@@ -591,7 +570,7 @@ __device__ __forceinline__ void raycast_pixel(const MapView& m, float* out, int 
     Walk wk{0, 0, 0, 0, false};
     do {
         if (wk.on) wk.on = walk_box(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, wk.bx, wk.by, wk.bz, wk.steps);
-        if (!wk.on) st = ray_step<FMT, RUNS>(r, rd, rtStep, &wk);
+        if (!wk.on) st = ray_step<FMT, RUNS>(r, rd, rtStep, wk);
     } while (st == kRayGoing);
     float4 v;
     if (st == kRaySurface && ray_surface(r, rd, Tinv, vsInv, rtStep, v)) *(float4*)out = v;

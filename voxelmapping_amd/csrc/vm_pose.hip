@@ -77,7 +77,7 @@ __device__ inline bool surface_depth(RayReader<FMT>& rd, int x, int y, const Int
     Walk wk{0, 0, 0, 0, false};
     do {
         if (wk.on) wk.on = walk_box(rd, r.res, r.dir, r.inv, r.dist_s, r.distEnd, wk.bx, wk.by, wk.bz, wk.steps);
-        if (!wk.on) st = ray_step(r, rd, rtStep, &wk);
+        if (!wk.on) st = ray_step(r, rd, rtStep, wk);
     } while (st == kRayGoing);
     if (st != kRaySurface) return false;
     float step = r.sdf * rtStep;
