@@ -14,7 +14,8 @@
 //
 // Backward, per pixel: the march is recomputed with the same code path as
 // the forward kernel, active steps are pushed to a per-pixel stack in HBM
-// (position, type, and the count of unit steps after it), and the reverse
+// (position, type, the count of unit steps after it, and the sample's value
+// and spatial gradient, so that the sweep reads no voxels), and the reverse
 // sweep applies lambda_k = lambda_{k+1} + (lambda_{k+1}.dir) rtStep grad s(r_k),
 // scattering (lambda.dir) rtStep w_j into dTSDF[j] with float atomics, and
 // accumulating d/d(dir), d/d(start) and the output transform into the pose
@@ -173,9 +174,14 @@ __device__ inline float3 vertex_grad(const float* gvin, int64_t gvpitch, int W, 
     return acc;
 }
 
+// An active step's sample: position, value and (trilinear samples) spatial
+// gradient. An active sample is valid, so a trilinear one read 8 observed
+// corners: its weights and voxel indices are all the sweep still needs.
 struct StackRec {
     float x, y, z;
     int meta;  // bit 0: interp sample; bits 1..: unit steps after this step
+    float value;
+    float gx, gy, gz;
 };
 
 // One wave per 8x8-pixel tile, tiles in the raycast's XCD-banded order; each
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             int depth = 0, lead = 0, units = 0;  // units: unit steps since the last pushed record
             // the last pushed record stays in registers until the unit steps
             // after it are known (the next push, or the end of the march)
-            StackRec pend{0.f, 0.f, 0.f, 0};
+            StackRec pend{0.f, 0.f, 0.f, 0, 0.f, 0.f, 0.f, 0.f};
             {
                 // unit steps below the near bound (ray_skip), counted as units
                 const float q = fminf(nearLim, distEnd - 2.0f) - dist_s;
@@ -283,7 +289,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 }
                 // the forward's sample (RayReader::sample), with its
                 // trilinear read taking the sample as its first corner
-                sdf = rd.sample(res, vx, vy, vz, p, valid);
+                sdf = rd.template sample<true>(res, vx, vy, vz, p, valid);
                 interpUsed = rd.lastInterp;
                 if (sdf <= 0.0f) break;
                 if (sdf >= 0.f && last < 0.f) break;  // the reference returns here: no hit
@@ -297,7 +303,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                         lead = units;
                     }
                     if (depth < kStackDepth) {
-                        pend = StackRec{res.x, res.y, res.z, interpUsed ? 1 : 0};
+                        const float3 g = rd.lastGrad;
+                        pend = StackRec{res.x, res.y, res.z, interpUsed ? 1 : 0, sdf, g.x, g.y, g.z};
                         ++depth;
                     } else {
                         overflow = true;
@@ -354,23 +361,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                         const StackRec rec = next;
                         if (d > 0) next = stack[(int64_t)(d - 1) * P + pix];
                         const float3 q = make_float3(rec.x, rec.y, rec.z);
-                        // the sample is read once: its value for d/d dir, its
-                        // weights and gradient for dTSDF and lambda
+                        // the record's value for d/d dir, its corner weights
+                        // and gradient for dTSDF and lambda
                         const float a = dot3f(lam, dir) * rtStep;
+                        gDir = add3(gDir, mul3(lam, rec.value * rtStep + (float)(rec.meta >> 1)));
                         if (rec.meta & 1) {
-                            Trilinear t;
-                            trilinear(rd, q, t);
-                            gDir = add3(gDir, mul3(lam, t.value * rtStep + (float)(rec.meta >> 1)));
+                            int64_t idx[8];
+                            float w[8];
+                            trilinear_corners(rd, q, idx);
+                            trilinear_weights(q.x - floorf(q.x), q.y - floorf(q.y), q.z - floorf(q.z), w);
 #pragma unroll
                             for (int k = 0; k < 8; ++k)
-                                if (t.ok[k] && t.w[k] != 0.f) acc.add(t.idx[k], a * t.w[k]);
-                            lam = add3(lam, mul3(t.grad, a));
+                                if (idx[k] >= 0 && w[k] != 0.f) acc.add(idx[k], a * w[k]);
+                            lam = add3(lam, mul3(make_float3(rec.gx, rec.gy, rec.gz), a));
                         } else {
-                            float v = 1.f;
-                            bool ok = false;
                             const int64_t i = rd.voxel(floor_i(q.x), floor_i(q.y), floor_i(q.z));
-                            if (i >= 0) Vox<kFmtF32>::read(m.voxels, i, v, ok);
-                            gDir = add3(gDir, mul3(lam, v * rtStep + (float)(rec.meta >> 1)));
                             if (i >= 0) acc.add(i, a);
                         }
                     }

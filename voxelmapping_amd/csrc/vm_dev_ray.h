@@ -55,10 +55,12 @@ struct RayReader {
     int qx, qy, qz;      // brick (block >> 2) of `word`; INT_MIN: none yet
     unsigned long long word;
     bool lastInterp;     // the last sample() took a trilinear read
+    float3 lastGrad;     // its spatial gradient (sample<true> only)
 
     __device__ explicit RayReader(const MapView& mv)
         : m(mv), cx(INT_MIN), cy(INT_MIN), cz(INT_MIN), cp(-1), px(INT_MIN), py(INT_MIN), pz(INT_MIN), pp(-1),
-          qx(INT_MIN), qy(INT_MIN), qz(INT_MIN), word(0), lastInterp(false) {}
+          qx(INT_MIN), qy(INT_MIN), qz(INT_MIN), word(0), lastInterp(false),
+          lastGrad(make_float3(0.f, 0.f, 0.f)) {}
 
     // false: no hash entry holds block (bx, by, bz)
     __device__ inline bool maybe_present(int bx, int by, int bz) {
@@ -139,7 +141,7 @@ struct RayReader {
                 : pb[4];
     }
 
-    template <bool HAVE0>
+    template <bool HAVE0, bool GRAD = false>
     __device__ __forceinline__ float interp_in(float3 q, int X0, int Y0, int Z0, int bx, int by, int bz, int p,
                                                float s0, bool ok0, bool& valid) {
         const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
@@ -169,6 +171,7 @@ struct RayReader {
             all = all && ok;
         }
         valid = all;
+        if (GRAD) lastGrad = blend_grad(fx, fy, fz, s);
         return blend(fx, fy, fz, s);
     }
 
@@ -190,11 +193,23 @@ struct RayReader {
         return (1.0f - fz) * r2 + fz * r3;
     }
 
+    // d blend / d q: the gradient of the same trilinear form (as trilinear()
+    // computes it)
+    __device__ static inline float3 blend_grad(float fx, float fy, float fz, const float s[8]) {
+        const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+        const float r0 = gx * s[0] + fx * s[1], r1 = gx * s[2] + fx * s[3];
+        const float q0 = gx * s[4] + fx * s[5], q1 = gx * s[6] + fx * s[7];
+        return make_float3(gz * (gy * (s[1] - s[0]) + fy * (s[3] - s[2])) + fz * (gy * (s[5] - s[4]) + fy * (s[7] - s[6])),
+                           gz * (r1 - r0) + fz * (q1 - q0), (gy * q0 + fy * q1) - (gy * r0 + fy * r1));
+    }
+
     // The march's sample at q, floor(q) = (vx, vy, vz) in block p (ptr or
     // -1): read_sdf of floor(q) (ray_tracing.cu:272-286: the voxel's value
     // when its weight is nonzero, else (1.0, invalid)), and
     // read_sdf_interped of q when that value is within [-0.5, 0.5]
     // (ray_tracing.cu:358-362).
+    // GRAD: a trilinear read also leaves its spatial gradient in lastGrad.
+    template <bool GRAD = false>
     __device__ inline float sample(float3 q, int vx, int vy, int vz, int p, bool& valid) {
         const int lx = vox_to_loc(vx), ly = vox_to_loc(vy), lz = vox_to_loc(vz);
         float sdf = 1.0f;
@@ -204,8 +219,8 @@ struct RayReader {
         if (!v) sdf = 1.0f;
         lastInterp = sdf <= 0.5f && sdf >= -0.5f;
         if (lastInterp)
-            sdf = interp_in<true>(q, vx, vy, vz, vox_to_blk(vx), vox_to_blk(vy), vox_to_blk(vz), p, sdf, valid,
-                                  valid);
+            sdf = interp_in<true, GRAD>(q, vx, vy, vz, vox_to_blk(vx), vox_to_blk(vy), vox_to_blk(vz), p, sdf,
+                                        valid, valid);
         return sdf;
     }
 };
@@ -227,9 +242,10 @@ struct Trilinear {
     bool valid;
 };
 
+// the voxel indices of the 8 corners of a trilinear read at q (-1: block
+// absent), as trilinear() resolves them
 template <int FMT>
-__device__ inline void trilinear(RayReader<FMT>& rd, float3 q, Trilinear& t) {
-    const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+__device__ inline void trilinear_corners(RayReader<FMT>& rd, float3 q, int64_t idx[8]) {
     const int X0 = floor_i(q.x), X1 = floor_i(q.x + 1.f);
     const int Y0 = floor_i(q.y), Y1 = floor_i(q.y + 1.f);
     const int Z0 = floor_i(q.z), Z1 = floor_i(q.z + 1.f);
@@ -242,10 +258,23 @@ __device__ inline void trilinear(RayReader<FMT>& rd, float3 q, Trilinear& t) {
         const int lz0 = vox_to_loc(Z0), lz1 = vox_to_loc(Z1);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            t.idx[k] = pb[k] < 0 ? -1
-                                 : (int64_t)pb[k] + local_index((k & 1) ? lx1 : lx0, (k & 2) ? ly1 : ly0,
-                                                                (k & 4) ? lz1 : lz0);
+            idx[k] = pb[k] < 0 ? -1
+                               : (int64_t)pb[k] + local_index((k & 1) ? lx1 : lx0, (k & 2) ? ly1 : ly0,
+                                                              (k & 4) ? lz1 : lz0);
     }
+}
+
+// the trilinear weights of the 8 corners at fractional offsets (fx, fy, fz)
+__device__ inline void trilinear_weights(float fx, float fy, float fz, float w[8]) {
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    w[0] = gx * gy * gz; w[1] = fx * gy * gz; w[2] = gx * fy * gz; w[3] = fx * fy * gz;
+    w[4] = gx * gy * fz; w[5] = fx * gy * fz; w[6] = gx * fy * fz; w[7] = fx * fy * fz;
+}
+
+template <int FMT>
+__device__ inline void trilinear(RayReader<FMT>& rd, float3 q, Trilinear& t) {
+    const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+    trilinear_corners(rd, q, t.idx);
     float s[8];
     t.valid = true;
 #pragma unroll
@@ -265,8 +294,7 @@ __device__ inline void trilinear(RayReader<FMT>& rd, float3 q, Trilinear& t) {
     const float q1 = gx * s[6] + fx * s[7];
     const float r3 = gy * q0 + fy * q1;
     t.value = gz * r2 + fz * r3;
-    t.w[0] = gx * gy * gz; t.w[1] = fx * gy * gz; t.w[2] = gx * fy * gz; t.w[3] = fx * fy * gz;
-    t.w[4] = gx * gy * fz; t.w[5] = fx * gy * fz; t.w[6] = gx * fy * fz; t.w[7] = fx * fy * fz;
+    trilinear_weights(fx, fy, fz, t.w);
     t.grad.x = gz * (gy * (s[1] - s[0]) + fy * (s[3] - s[2])) + fz * (gy * (s[5] - s[4]) + fy * (s[7] - s[6]));
     t.grad.y = gz * (r1 - r0) + fz * (q1 - q0);
     t.grad.z = r3 - r2;
