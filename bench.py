@@ -246,7 +246,7 @@ def pose_bench(m, w, depth, T, n_hyp: int = 32, iters: int = 8, stride: int = 4,
     dT = np.linalg.inv(T0) @ poses[best].astype(np.float64)
     return {"workload": f"{n_hyp} hypotheses x {iters} Gauss-Newton iterations, 640x480 depth every {stride}th pixel, "
                         "room_vga map", "ms_per_refinement": round(1000 * dt, 3),
-            "rays_per_s": round(rays / dt, 1), "launches_per_iteration": 1,
+            "rays_per_s": round(rays / dt, 1), "launches_per_iteration": 3,
             "best_hypothesis_error_mm": round(1000 * float(np.linalg.norm(dT[:3, 3])), 2)}
 
 
@@ -257,8 +257,8 @@ def scale_bench(frames: int = 40, warmup: int = 4, track_every: int = 10, n_hyp:
     raycast + normals per frame. Every track_every-th frame, before it is
     fused, n_hyp pose hypotheses around its true pose (U[-2, 2] cm,
     U[-1.5, 1.5] deg) are refined against its depth by iters Gauss-Newton
-    iterations over every stride-th pixel (one launch per iteration for all
-    hypotheses); refinement time is reported apart from the fusion step.
+    iterations over every stride-th pixel (one Gauss-Newton launch per
+    iteration for all hypotheses); refinement time is reported apart from the fusion step.
     Tracking uses the depth below track_max_depth metres (0: all): the
     workload's noise, 0.0012 z^2 m, stays below half its 1 cm truncation up to
     2 m, and beyond that the fused surface is biased by one-sided averaging.

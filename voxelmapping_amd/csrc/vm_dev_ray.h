@@ -685,6 +685,64 @@ struct BoundsShared {
     int nHeavy;
 };
 
+// Reach of one allocated block (b.x != kEmptyKey) from a camera at Tinv
+// (centre c, voxel units) for a W x H image: the distance range [near, far]
+// of its box of march positions from c (float bits), and where rays can
+// reach it: kReachNone (behind z = 5 cm or beyond z = 5.05 m), kReachAll (the
+// box reaches z < 1 cm: any ray), kReachRect (the pixels [x0, x1] x [y0, y1],
+// the hull of its corners' projections grown by a pixel, clipped to the
+// image; not written when the rectangle misses the image: kReachNone).
+enum BlockReach : int { kReachNone = 0, kReachAll = 1, kReachRect = 2 };
+struct Reach {
+    int kind;
+    unsigned nearBits, farBits;
+    int x0, x1, y0, y1;
+};
+__device__ __forceinline__ Reach block_reach(const int4 b, int W, int H, const Intrinsics& K, const Affine& Tinv,
+                                             float3 c, float vs, float grow) {
+    Reach r{kReachNone, 0u, 0u, 0, -1, 0, -1};
+    const float lx = (float)(b.x * 8) - 1.0f - grow, hx = (float)(b.x * 8 + 8) + grow;
+    const float ly = (float)(b.y * 8) - 1.0f - grow, hy = (float)(b.y * 8 + 8) + grow;
+    const float lz = (float)(b.z * 8) - 1.0f - grow, hz = (float)(b.z * 8 + 8) + grow;
+    // distance range of the box from the camera centre
+    const float nx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.0f);
+    const float ny = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.0f);
+    const float nz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.0f);
+    const float ex = fmaxf(fabsf(lx - c.x), fabsf(hx - c.x));
+    const float ey = fmaxf(fabsf(ly - c.y), fabsf(hy - c.y));
+    const float ez = fmaxf(fabsf(lz - c.z), fabsf(hz - c.z));
+    r.nearBits = __float_as_uint(sqrtf(nx * nx + ny * ny + nz * nz));
+    r.farBits = __float_as_uint(sqrtf(ex * ex + ey * ey + ez * ez));
+    float zmin = INFINITY, zmax = -INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float3 w = make_float3(((k & 1) ? hx : lx) * vs, ((k & 2) ? hy : ly) * vs, ((k & 4) ? hz : lz) * vs);
+        const float3 p = xform(Tinv, w);
+        zmin = fminf(zmin, p.z);
+        zmax = fmaxf(zmax, p.z);
+        // hardware reciprocal (1 ulp): u and v move by a few 2^-23 of their
+        // size, far below the one pixel the rectangle is grown by wherever
+        // they fall near the image (farther out the rectangle is clipped to
+        // the image anyway)
+        const float iz = __builtin_amdgcn_rcpf(p.z);
+        const float u = K.fx * p.x * iz + K.cx, v = K.fy * p.y * iz + K.cy;
+        umin = fminf(umin, u); umax = fmaxf(umax, u);
+        vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+    }
+    if (!(zmax >= 0.05f && zmin <= 5.05f)) return r;
+    if (zmin < 0.01f) {
+        r.kind = kReachAll;
+        return r;
+    }
+    const float x0 = fmaxf(floorf(umin) - 1.0f, 0.0f), x1 = fminf(ceilf(umax) + 1.0f, (float)(W - 1));
+    const float y0 = fmaxf(floorf(vmin) - 1.0f, 0.0f), y1 = fminf(ceilf(vmax) + 1.0f, (float)(H - 1));
+    if (x0 <= x1 && y0 <= y1) {
+        r.kind = kReachRect;
+        r.x0 = (int)x0; r.x1 = (int)x1; r.y0 = (int)y0; r.y1 = (int)y1;
+    }
+    return r;
+}
+
 // Bounds of the allocated blocks [base, base + CH) of map m, thread i < CH
 // taking block base + i, by a workgroup of at least CH threads (all of which
 // share the large rectangles). A block's box of march positions whose sample can read one
@@ -710,59 +768,23 @@ __device__ __forceinline__ void ray_bounds_blocks(const int4 b, const RayBounds&
     unsigned long long* farK = rb.keys + rb.nTiles + 1;
     if (tid == 0) sh.nHeavy = 0;
     __syncthreads();
-    {
-        if (b.x != kEmptyKey) {
-            const float lx = (float)(b.x * 8) - 1.0f - grow, hx = (float)(b.x * 8 + 8) + grow;
-            const float ly = (float)(b.y * 8) - 1.0f - grow, hy = (float)(b.y * 8 + 8) + grow;
-            const float lz = (float)(b.z * 8) - 1.0f - grow, hz = (float)(b.z * 8 + 8) + grow;
-            // distance range of the box from the camera centre
-            const float nx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.0f);
-            const float ny = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.0f);
-            const float nz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.0f);
-            const float ex = fmaxf(fabsf(lx - c.x), fabsf(hx - c.x));
-            const float ey = fmaxf(fabsf(ly - c.y), fabsf(hy - c.y));
-            const float ez = fmaxf(fabsf(lz - c.z), fabsf(hz - c.z));
-            const unsigned nb = __float_as_uint(sqrtf(nx * nx + ny * ny + nz * nz));
-            const unsigned fb = __float_as_uint(sqrtf(ex * ex + ey * ey + ez * ez));
-            float zmin = INFINITY, zmax = -INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY,
-                  vmax = -INFINITY;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float3 w =
-                    make_float3(((k & 1) ? hx : lx) * vs, ((k & 2) ? hy : ly) * vs, ((k & 4) ? hz : lz) * vs);
-                const float3 p = xform(Tinv, w);
-                zmin = fminf(zmin, p.z);
-                zmax = fmaxf(zmax, p.z);
-                // hardware reciprocal (1 ulp): u and v move by a few 2^-23
-                // of their size, far below the one pixel the rectangle is
-                // grown by wherever they fall near the image (farther out the
-                // rectangle is clipped to the image anyway)
-                const float iz = __builtin_amdgcn_rcpf(p.z);
-                const float u = K.fx * p.x * iz + K.cx, v = K.fy * p.y * iz + K.cy;
-                umin = fminf(umin, u); umax = fmaxf(umax, u);
-                vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
-            }
-            if (zmax >= 0.05f && zmin <= 5.05f) {
-                if (zmin < 0.01f) {
-                    atomicMin(&nearK[rb.nTiles], ne | nb);
-                    atomicMax(&farK[rb.nTiles], fe | fb);
-                } else {
-                    const float x0 = fmaxf(floorf(umin) - 1.0f, 0.0f), x1 = fminf(ceilf(umax) + 1.0f, (float)(W - 1));
-                    const float y0 = fmaxf(floorf(vmin) - 1.0f, 0.0f), y1 = fminf(ceilf(vmax) + 1.0f, (float)(H - 1));
-                    if (x0 <= x1 && y0 <= y1) {
-                        const int tx0 = (int)x0 >> 3, tx1 = (int)x1 >> 3, ty0 = (int)y0 >> 3, ty1 = (int)y1 >> 3;
-                        const int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-                        if (n <= kBoundsLightTiles) {
-                            for (int ty = ty0; ty <= ty1; ++ty)
-                                for (int tx = tx0; tx <= tx1; ++tx) {
-                                    atomicMin(&nearK[ty * tilesX + tx], ne | nb);
-                                    atomicMax(&farK[ty * tilesX + tx], fe | fb);
-                                }
-                        } else {
-                            sh.heavy[atomicAdd(&sh.nHeavy, 1)] = TileRect{tx0, ty0, tx1 - tx0 + 1, n, nb, fb};
-                        }
+    if (b.x != kEmptyKey) {
+        const Reach r = block_reach(b, W, H, K, Tinv, c, vs, grow);
+        const unsigned nb = r.nearBits, fb = r.farBits;
+        if (r.kind == kReachAll) {
+            atomicMin(&nearK[rb.nTiles], ne | nb);
+            atomicMax(&farK[rb.nTiles], fe | fb);
+        } else if (r.kind == kReachRect) {
+            const int tx0 = r.x0 >> 3, tx1 = r.x1 >> 3, ty0 = r.y0 >> 3, ty1 = r.y1 >> 3;
+            const int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+            if (n <= kBoundsLightTiles) {
+                for (int ty = ty0; ty <= ty1; ++ty)
+                    for (int tx = tx0; tx <= tx1; ++tx) {
+                        atomicMin(&nearK[ty * tilesX + tx], ne | nb);
+                        atomicMax(&farK[ty * tilesX + tx], fe | fb);
                     }
-                }
+            } else {
+                sh.heavy[atomicAdd(&sh.nHeavy, 1)] = TileRect{tx0, ty0, tx1 - tx0 + 1, n, nb, fb};
             }
         }
     }

@@ -1,6 +1,7 @@
 // Batched pose refinement: Gauss-Newton on the depth residual
 // z(T) - depth over a pixel grid, for many camera-pose hypotheses in one
-// launch per iteration. No counterpart in the reference; it builds on its
+// Gauss-Newton launch per iteration (after a small launch that bounds each
+// hypothesis's rays). No counterpart in the reference; it builds on its
 // raycast (MapRenderingDelegate, ray_tracing.cu:323-400).
 //
 // z(T) of a pixel is the depth of the map's zero level along the pixel's ray:
@@ -65,12 +66,19 @@ constexpr int kZeroSteps = 3;
 
 // Depth of pixel (x, y) at pose T and its derivative by the pose twist.
 // Returns false when the raycast misses or the zero level is not found.
+// [nearLim, farLim] brackets every sample of any ray of the image that can
+// read a voxel of an allocated block (k_pose_bounds), as the raycast's tile
+// bounds do (raycast_pixel): the samples before nearLim are the same
+// additions without reads, and a ray reaching farLim misses.
 template <int FMT>
 __device__ inline bool surface_depth(RayReader<FMT>& rd, int x, int y, const Intrinsics& K, const Affine& T,
-                                     const Affine& Tinv, float vsInv, float rtStep, float& z, float J[6]) {
+                                     const Affine& Tinv, float vsInv, float rtStep, float nearLim, float farLim,
+                                     float& z, float J[6]) {
     Ray r;
     ray_begin(r, x, y, K, T, vsInv);
     const float3 o = r.res;
+    r.distEnd = fminf(r.distEnd, farLim);
+    ray_skip(r, nearLim);
     // the raycast's march loop: walks through empty blocks one box per pass
     // (raycast_pixel, vm_dev_ray.h)
     int st = kRayGoing;
@@ -128,13 +136,99 @@ __device__ inline bool surface_depth(RayReader<FMT>& rd, int x, int y, const Int
     return true;
 }
 
+// Per hypothesis, the range of ray distance (voxels along the ray, the unit
+// of the march's dist_s) in which any ray of the W x H image can read a
+// voxel of an allocated block: the union of the raycast's per-tile ranges
+// (block_reach over every allocated block, boxes grown by ray_drift_margin).
+// Workgroup (w, h) takes blocks w * 256 + i, w * 256 + i + 256 * gridDim.x,
+// ... for hypothesis h and writes its (near, far) float bits to
+// bounds[2 (h gridDim.x + w)]; the march takes the minimum and maximum over
+// the workgroups. A hypothesis whose rotation is not orthonormal (an entry
+// of R^T R - I above 1e-5) gets no bounds (0, +inf).
+constexpr int kPoseBoundsThreads = 256;
+constexpr int kPoseBoundsGroups = 64;
+__global__ __launch_bounds__(kPoseBoundsThreads) void k_pose_bounds(MapView m, int W, int H, Intrinsics K,
+                                                                    const Affine* poses, const Affine* posesInv,
+                                                                    unsigned* bounds) {
+    __shared__ unsigned s_near[kPoseBoundsThreads / 64], s_far[kPoseBoundsThreads / 64];
+    const int h = blockIdx.y;
+    const Affine T = poses[h], Ti = posesInv[h];
+    const float vs = m.voxelSize, vsInv = (float)(1.0 / (double)m.voxelSize);
+    const float3 c = make_float3(T.t0 * vsInv, T.t1 * vsInv, T.t2 * vsInv);
+    const float grow = ray_drift_margin(W, H, K, T.t0, T.t1, T.t2, vsInv);
+    const int A = allocated_count(m.ctr, m.nBlock);
+    unsigned nb = __float_as_uint(INFINITY), fb = 0u;
+    for (int i = blockIdx.x * kPoseBoundsThreads + threadIdx.x; i < A; i += gridDim.x * kPoseBoundsThreads) {
+        const int4 b = m.blockPos[i];
+        if (b.x == kEmptyKey) continue;
+        const Reach r = block_reach(b, W, H, K, Ti, c, vs, grow);
+        if (r.kind != kReachNone) {
+            nb = min(nb, r.nearBits);  // non-negative floats order as their bits
+            fb = max(fb, r.farBits);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        nb = min(nb, (unsigned)__shfl_xor((int)nb, o, 64));
+        fb = max(fb, (unsigned)__shfl_xor((int)fb, o, 64));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_near[wave] = nb;
+        s_far[wave] = fb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kPoseBoundsThreads / 64; ++w) {
+            nb = min(nb, s_near[w]);
+            fb = max(fb, s_far[w]);
+        }
+        const float rr[3][3] = {{T.r00, T.r01, T.r02}, {T.r10, T.r11, T.r12}, {T.r20, T.r21, T.r22}};
+        bool rigid = true;
+        for (int a = 0; a < 3; ++a)
+            for (int e = 0; e < 3; ++e) {
+                const double dot = (double)rr[0][a] * rr[0][e] + (double)rr[1][a] * rr[1][e] + (double)rr[2][a] * rr[2][e];
+                rigid = rigid && fabs(dot - (a == e ? 1.0 : 0.0)) <= 1e-5;
+            }
+        if (!rigid) {
+            nb = 0u;
+            fb = __float_as_uint(INFINITY);
+        }
+        bounds[2 * ((int64_t)h * gridDim.x + blockIdx.x)] = nb;
+        bounds[2 * ((int64_t)h * gridDim.x + blockIdx.x) + 1] = fb;
+    }
+}
+
 // one workgroup: kGnThreads sampled pixels of hypothesis blockIdx.y
 template <int FMT, int DF>
 __global__ __launch_bounds__(kGnThreads) void k_pose_gn(MapView m, DepthImage d, Intrinsics K, const Affine* poses,
                                                         const Affine* posesInv, int stride, int sw, int sh,
-                                                        float vsInv, float rtStep, float maxRes, float* partials) {
+                                                        float vsInv, float rtStep, float maxRes,
+                                                        const unsigned* bounds, float* partials) {
     __shared__ float s_part[kGnThreads / 64][kGnTerms];
+    __shared__ unsigned s_lim[2];
     const int h = blockIdx.y;
+    // the hypothesis's ray-distance range, from k_pose_bounds's partials
+    static_assert(kPoseBoundsGroups <= 64 && kGnThreads >= 64, "one wave reduces the bounds partials");
+    if (threadIdx.x < 64) {
+        unsigned nb = __float_as_uint(INFINITY), fb = 0u;
+        if (threadIdx.x < kPoseBoundsGroups) {
+            nb = bounds[2 * ((int64_t)h * kPoseBoundsGroups + threadIdx.x)];
+            fb = bounds[2 * ((int64_t)h * kPoseBoundsGroups + threadIdx.x) + 1];
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            nb = min(nb, (unsigned)__shfl_xor((int)nb, o, 64));
+            fb = max(fb, (unsigned)__shfl_xor((int)fb, o, 64));
+        }
+        if (threadIdx.x == 0) {
+            s_lim[0] = nb;
+            s_lim[1] = fb;
+        }
+    }
+    __syncthreads();
+    const float margin = ray_drift_margin(d.width, d.height, K, poses[h].t0, poses[h].t1, poses[h].t2, vsInv);
+    const float nearLim = __uint_as_float(s_lim[0]) - margin, farLim = __uint_as_float(s_lim[1]) + margin;
     const int i = blockIdx.x * kGnThreads + threadIdx.x;
     float acc[kGnTerms];
 #pragma unroll
@@ -146,7 +240,7 @@ __global__ __launch_bounds__(kGnThreads) void k_pose_gn(MapView m, DepthImage d,
             acc[29] = 1.f;
             RayReader<FMT> rd(m);
             float z, J[6];
-            if (surface_depth(rd, x, y, K, poses[h], posesInv[h], vsInv, rtStep, z, J)) {
+            if (surface_depth(rd, x, y, K, poses[h], posesInv[h], vsInv, rtStep, nearLim, farLim, z, J)) {
                 const float res = z - dep;
                 if (fabsf(res) < maxRes) {
                     int k = 0;
@@ -341,17 +435,19 @@ __global__ void k_pose_load(const double* posesD, Affine* poses, Affine* posesIn
 
 template <int FMT>
 hipError_t launch_gn(const MapView& m, const DepthImage& d, const Intrinsics& K, const Affine* poses,
-                     const Affine* posesInv, int nHyp, int stride, int sw, int sh, float maxRes, float* partials,
-                     int blocksPerHyp, hipStream_t s) {
+                     const Affine* posesInv, int nHyp, int stride, int sw, int sh, float maxRes, unsigned* bounds,
+                     float* partials, int blocksPerHyp, hipStream_t s) {
     const float vsInv = (float)(1.0 / (double)m.voxelSize);
     const float rtStep = m.trunc / m.voxelSize;
+    hipLaunchKernelGGL(k_pose_bounds, dim3(kPoseBoundsGroups, nHyp), dim3(kPoseBoundsThreads), 0, s, m, d.width,
+                       d.height, K, poses, posesInv, bounds);
     const dim3 grid(blocksPerHyp, nHyp);
     if (d.format == kDepthU16)
         hipLaunchKernelGGL((k_pose_gn<FMT, kDepthU16>), grid, dim3(kGnThreads), 0, s, m, d, K, poses, posesInv,
-                           stride, sw, sh, vsInv, rtStep, maxRes, partials);
+                           stride, sw, sh, vsInv, rtStep, maxRes, bounds, partials);
     else
         hipLaunchKernelGGL((k_pose_gn<FMT, kDepthF32>), grid, dim3(kGnThreads), 0, s, m, d, K, poses, posesInv,
-                           stride, sw, sh, vsInv, rtStep, maxRes, partials);
+                           stride, sw, sh, vsInv, rtStep, maxRes, bounds, partials);
     return hipGetLastError();
 }
 
@@ -363,7 +459,8 @@ size_t refine_poses_scratch_bytes(int W, int H, int nHyp, int iters, int stride)
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     return up((size_t)nHyp * bph * kGnTerms * sizeof(float)) + 2 * up((size_t)nHyp * 16 * sizeof(double)) +
            up((size_t)nHyp * kLmState * sizeof(double)) + 2 * up((size_t)nHyp * sizeof(Affine)) +
-           up((size_t)nHyp * (iters + 1) * 2 * sizeof(float)) + up((size_t)nHyp * (iters + 1) * 27 * sizeof(double));
+           up((size_t)nHyp * (iters + 1) * 2 * sizeof(float)) + up((size_t)nHyp * (iters + 1) * 27 * sizeof(double)) +
+           up((size_t)nHyp * kPoseBoundsGroups * 2 * sizeof(unsigned));
 }
 
 // posesIn: nHyp row-major 4x4 doubles (host). scratch: refine_poses_scratch_bytes.
@@ -392,15 +489,17 @@ hipError_t launch_refine_poses(const MapView& m, const DepthImage& d, const Intr
     float* cost = (float*)p;
     p += up((size_t)nHyp * (iters + 1) * 2 * sizeof(float));
     double* system = (double*)p;
+    p += up((size_t)nHyp * (iters + 1) * 27 * sizeof(double));
+    unsigned* bounds = (unsigned*)p;
     hipError_t e = hipMemcpyAsync(posesD, posesIn, (size_t)nHyp * 16 * sizeof(double), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     const int tb = 64, nb = (nHyp + tb - 1) / tb;
     hipLaunchKernelGGL(k_pose_load, dim3(nb), dim3(tb), 0, s, posesD, poses, posesInv, nHyp);
     for (int it = 0; it <= iters; ++it) {
         switch (m.fmt) {
-            case kFmtRef: e = launch_gn<kFmtRef>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, partials, bph, s); break;
-            case kFmtRefRGB: e = launch_gn<kFmtRefRGB>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, partials, bph, s); break;
-            default: e = launch_gn<kFmtF32>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, partials, bph, s);
+            case kFmtRef: e = launch_gn<kFmtRef>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, bounds, partials, bph, s); break;
+            case kFmtRefRGB: e = launch_gn<kFmtRefRGB>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, bounds, partials, bph, s); break;
+            default: e = launch_gn<kFmtF32>(m, d, K, poses, posesInv, nHyp, stride, sw, sh, maxRes, bounds, partials, bph, s);
         }
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_pose_solve, dim3(nHyp), dim3(kSolveThreads), 0, s, partials, bph, nHyp, posesD, poses,
