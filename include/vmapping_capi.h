@@ -208,7 +208,9 @@ int vm_surface_points(const float* verts, float* points, int64_t n_triangles, vo
  *   grad_pose[k] += d loss / d xi_k (device, 6 doubles) for
  *                   camToWorld = T exp(xi), xi = (rho_x, rho_y, rho_z, phi_x, phi_y, phi_z).
  * The march is recomputed with its discrete decisions (cells, interpolation
- * test, break step) held fixed; memory is O(pixels). Float voxel format only.
+ * test, break step) held fixed; memory is O(pixels): a work buffer of about
+ * 1.5 KiB per pixel (48 records of 32 B), allocated by the first call that
+ * needs it and kept until vm_map_destroy. Float voxel format only.
  * Pixels whose march has more than 48 value-dependent steps contribute no
  * gradient and are counted in overflow_pixels; the counts are kept only when
  * out_stats is given (the call then waits for them). */
