@@ -88,7 +88,7 @@ def cpu_baseline(w, seconds: float, max_frames: int = 64) -> dict:
     first frames of the same sequence -- frame i fused at pose i, then the
     raycast and normals at pose i, exactly the GPU step -- until `seconds` of
     work or max_frames frames are done. Threads: the host's CPU share of one
-    GPU, at most 16 (what a one-GPU job may use on a shared box), below
+    GPU, at most 16 (what a one-GPU job may use on a shared machine), below
     that the cores this process may run on."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
