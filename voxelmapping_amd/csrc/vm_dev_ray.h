@@ -698,26 +698,47 @@ struct Reach {
     unsigned nearBits, farBits;
     int x0, x1, y0, y1;
 };
-__device__ __forceinline__ Reach block_reach(const int4 b, int W, int H, const Intrinsics& K, const Affine& Tinv,
-                                             float3 c, float vs, float grow) {
-    Reach r{kReachNone, 0u, 0u, 0, -1, 0, -1};
+// squared distance range [near2, far2] of the box of march positions of
+// block b (grown by `grow`) from the point c (voxel units)
+struct BoxDist2 {
+    float near2, far2;
+};
+__device__ __forceinline__ BoxDist2 block_dist2(const int4 b, float3 c, float grow) {
     const float lx = (float)(b.x * 8) - 1.0f - grow, hx = (float)(b.x * 8 + 8) + grow;
     const float ly = (float)(b.y * 8) - 1.0f - grow, hy = (float)(b.y * 8 + 8) + grow;
     const float lz = (float)(b.z * 8) - 1.0f - grow, hz = (float)(b.z * 8 + 8) + grow;
-    // distance range of the box from the camera centre
     const float nx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.0f);
     const float ny = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.0f);
     const float nz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.0f);
     const float ex = fmaxf(fabsf(lx - c.x), fabsf(hx - c.x));
     const float ey = fmaxf(fabsf(ly - c.y), fabsf(hy - c.y));
     const float ez = fmaxf(fabsf(lz - c.z), fabsf(hz - c.z));
-    r.nearBits = __float_as_uint(sqrtf(nx * nx + ny * ny + nz * nz));
-    r.farBits = __float_as_uint(sqrtf(ex * ex + ey * ey + ez * ez));
+    return BoxDist2{nx * nx + ny * ny + nz * nz, ex * ex + ey * ey + ez * ez};
+}
+
+// where rays can reach the box of block b (Reach::kind and the pixel
+// rectangle; the distance fields are left as they are)
+__device__ __forceinline__ void block_reach_kind(const int4 b, int W, int H, const Intrinsics& K, const Affine& Tinv,
+                                                 float vs, float grow, Reach& r) {
+    const float lx = (float)(b.x * 8) - 1.0f - grow, hx = (float)(b.x * 8 + 8) + grow;
+    const float ly = (float)(b.y * 8) - 1.0f - grow;
+    const float lz = (float)(b.z * 8) - 1.0f - grow;
     float zmin = INFINITY, zmax = -INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+    // the corners in the camera frame: the low corner's transform plus the
+    // box's edges along the rotated world axes (a few ulps from transforming
+    // each corner, far inside the pixel the rectangle is grown by and the
+    // centimetres of the depth tests)
+    const float3 p0 = xform(Tinv, make_float3(lx * vs, ly * vs, lz * vs));
+    const float E = (hx - lx) * vs;  // the same edge on every axis
+    const float3 dx = make_float3(Tinv.r00 * E, Tinv.r10 * E, Tinv.r20 * E);
+    const float3 dy = make_float3(Tinv.r01 * E, Tinv.r11 * E, Tinv.r21 * E);
+    const float3 dz = make_float3(Tinv.r02 * E, Tinv.r12 * E, Tinv.r22 * E);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const float3 w = make_float3(((k & 1) ? hx : lx) * vs, ((k & 2) ? hy : ly) * vs, ((k & 4) ? hz : lz) * vs);
-        const float3 p = xform(Tinv, w);
+        float3 p = p0;
+        if (k & 1) p = make_float3(p.x + dx.x, p.y + dx.y, p.z + dx.z);
+        if (k & 2) p = make_float3(p.x + dy.x, p.y + dy.y, p.z + dy.z);
+        if (k & 4) p = make_float3(p.x + dz.x, p.y + dz.y, p.z + dz.z);
         zmin = fminf(zmin, p.z);
         zmax = fmaxf(zmax, p.z);
         // hardware reciprocal (1 ulp): u and v move by a few 2^-23 of their
@@ -729,10 +750,11 @@ __device__ __forceinline__ Reach block_reach(const int4 b, int W, int H, const I
         umin = fminf(umin, u); umax = fmaxf(umax, u);
         vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
     }
-    if (!(zmax >= 0.05f && zmin <= 5.05f)) return r;
+    r.kind = kReachNone;
+    if (!(zmax >= 0.05f && zmin <= 5.05f)) return;
     if (zmin < 0.01f) {
         r.kind = kReachAll;
-        return r;
+        return;
     }
     const float x0 = fmaxf(floorf(umin) - 1.0f, 0.0f), x1 = fminf(ceilf(umax) + 1.0f, (float)(W - 1));
     const float y0 = fmaxf(floorf(vmin) - 1.0f, 0.0f), y1 = fminf(ceilf(vmax) + 1.0f, (float)(H - 1));
@@ -740,6 +762,15 @@ __device__ __forceinline__ Reach block_reach(const int4 b, int W, int H, const I
         r.kind = kReachRect;
         r.x0 = (int)x0; r.x1 = (int)x1; r.y0 = (int)y0; r.y1 = (int)y1;
     }
+}
+
+__device__ __forceinline__ Reach block_reach(const int4 b, int W, int H, const Intrinsics& K, const Affine& Tinv,
+                                             float3 c, float vs, float grow) {
+    Reach r{kReachNone, 0u, 0u, 0, -1, 0, -1};
+    const BoxDist2 d = block_dist2(b, c, grow);
+    r.nearBits = __float_as_uint(sqrtf(d.near2));
+    r.farBits = __float_as_uint(sqrtf(d.far2));
+    block_reach_kind(b, W, H, K, Tinv, vs, grow, r);
     return r;
 }
 

@@ -157,21 +157,48 @@ __global__ __launch_bounds__(kPoseBoundsThreads) void k_pose_bounds(MapView m, i
     const float3 c = make_float3(T.t0 * vsInv, T.t1 * vsInv, T.t2 * vsInv);
     const float grow = ray_drift_margin(W, H, K, T.t0, T.t1, T.t2, vsInv);
     const int A = allocated_count(m.ctr, m.nBlock);
-    unsigned nb = __float_as_uint(INFINITY), fb = 0u;
-    for (int i = blockIdx.x * kPoseBoundsThreads + threadIdx.x; i < A; i += gridDim.x * kPoseBoundsThreads) {
-        const int4 b = m.blockPos[i];
-        if (b.x == kEmptyKey) continue;
-        const Reach r = block_reach(b, W, H, K, Ti, c, vs, grow);
-        if (r.kind != kReachNone) {
-            nb = min(nb, r.nearBits);  // non-negative floats order as their bits
-            fb = max(fb, r.farBits);
+    // squared distances, kept only for blocks the rays reach; a block whose
+    // range lies inside the range kept so far cannot change it, so its
+    // projection is skipped
+    float n2 = INFINITY, f2 = 0.0f;
+    const int stride = gridDim.x * kPoseBoundsThreads;
+    // four blocks per pass, their positions loaded together
+    for (int i0 = blockIdx.x * kPoseBoundsThreads + threadIdx.x; i0 < A; i0 += 4 * stride) {
+        int4 bq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            bq[q] = i0 + q * stride < A ? m.blockPos[i0 + q * stride] : make_int4(kEmptyKey, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 b = bq[q];
+            if (b.x == kEmptyKey) continue;
+            const BoxDist2 d = block_dist2(b, c, grow);
+            if (d.near2 < n2 || d.far2 > f2) {
+                Reach r;
+                block_reach_kind(b, W, H, K, Ti, vs, grow, r);
+                if (r.kind != kReachNone) {
+                    n2 = fminf(n2, d.near2);
+                    f2 = fmaxf(f2, d.far2);
+                }
+            }
+        }
+        // the wave's range so far in every lane: a lane then projects only
+        // the blocks that could widen the wave's range, and the wave skips
+        // the projection when none of its lanes has such a block
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            n2 = fminf(n2, __shfl_xor(n2, o, 64));
+            f2 = fmaxf(f2, __shfl_xor(f2, o, 64));
         }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        nb = min(nb, (unsigned)__shfl_xor((int)nb, o, 64));
-        fb = max(fb, (unsigned)__shfl_xor((int)fb, o, 64));
+        n2 = fminf(n2, __shfl_xor(n2, o, 64));
+        f2 = fmaxf(f2, __shfl_xor(f2, o, 64));
     }
+    // the rounded square roots of the extremes are the extremes of the
+    // rounded square roots (sqrt is monotone): the raycast's per-block values
+    unsigned nb = __float_as_uint(sqrtf(n2)), fb = __float_as_uint(sqrtf(f2));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
         s_near[wave] = nb;
