@@ -223,8 +223,9 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
                         int32_t height, const float K[4], const float cam_to_world[16], float* grad_tsdf,
                         double* grad_pose, void* stream, vm_backward_stats* out_stats);
 
-/* Batched pose refinement (no counterpart in the reference): n_hyp
- * camera-to-world hypotheses (HOST, column-major float[16] each) are refined
+/* Batched pose refinement (no counterpart in the reference): n_hyp rigid
+ * camera-to-world hypotheses (HOST, column-major float[16] each; every entry
+ * of R^T R - I within 1e-5, else VM_ERR_INVALID_ARG) are refined
  * against one depth image (device, formats as vm_fuse_depth) by `iters`
  * Gauss-Newton iterations on the depth residual r = z_render(T) - depth over
  * pixels (x, y) with x and y multiples of `stride`. z_render is the camera z
@@ -236,8 +237,10 @@ int vm_raycast_backward(vm_map* map, const float* vmap, int64_t vmap_pitch, cons
  * min(depth_max, max_depth)] are left out (max_depth <= 0: depth_max) --
  * with depth noise that grows with range, a max_depth where the noise stays
  * below half the truncation keeps the fit on the part of the map that
- * fusion did not bias. Each iteration is one launch
- * over all hypotheses, then a Levenberg-Marquardt decision per hypothesis: a
+ * fusion did not bias. Each iteration is one Gauss-Newton launch over all
+ * hypotheses (after a small launch that bounds each hypothesis's ray
+ * distances, as vm_raycast's tile bounds do; results are the same bit for
+ * bit), then a Levenberg-Marquardt decision per hypothesis: a
  * new pose is kept only if it does not raise the truncated cost (sum of r^2
  * below max_residual, max_residual^2 for every other pixel with a valid
  * depth, over the valid pixels), else the previous pose is restored and the

@@ -198,6 +198,11 @@ def test_arguments_are_checked(fused):
     bad[0, 0, 0] = np.nan
     with pytest.raises(VMError):
         g.refine_poses(d, w.K, bad)
+    # hypotheses are rigid: a rotation scaled by 1.001 is refused
+    scaled = T.copy()
+    scaled[0, :3, :3] *= np.float32(1.001)
+    with pytest.raises(VMError):
+        g.refine_poses(d, w.K, scaled)
     # an empty depth image leaves every pose as it was and reports no residuals
     poses, cost = g.refine_poses(d, w.K, T, iters=2)
     assert np.array_equal(poses, T) and np.all(cost[..., 1] == 0) and np.all(cost[..., 0] == -1)

@@ -746,6 +746,9 @@ int vm_refine_poses(vm_map* m, const void* depth, int64_t pitch, int32_t W, int3
     std::vector<double> Tin((size_t)n_hyp * 16);
     for (int h = 0; h < n_hyp; ++h) {
         if (int rc = check_pose_k(K, init_poses + (size_t)h * 16)) return rc;
+        // the refinement inverts hypotheses as [R^T, -R^T t]
+        if (!rigid_rotation(init_poses + (size_t)h * 16))
+            return fail(VM_ERR_INVALID_ARG, "pose hypotheses must be rigid (orthonormal rotation)");
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) Tin[(size_t)h * 16 + i * 4 + j] = init_poses[(size_t)h * 16 + j * 4 + i];
     }
