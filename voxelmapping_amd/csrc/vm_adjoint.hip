@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             // the ray-distance bounds of the tile end the march early as the
             // forward raycast does (raycast_pixel); a miss has no adjoint
             float nearLim, farLim;
-            ray_bounds_limits(rb, tile, margin, nearLim, farLim);
+            ray_bounds_limits(rb, bounds_tile(x, y, W), margin, nearLim, farLim);
             const float distEnd = fminf(norm3(pf) * vsInv, farLim);
             float3 be = xform(pose, pf);
             be = make_float3(be.x * vsInv, be.y * vsInv, be.z * vsInv);

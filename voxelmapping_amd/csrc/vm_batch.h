@@ -24,7 +24,7 @@ struct BatchScene {
     int tile16First;          // first 16x16 tile of this scene in the flat tile space
     int tilesX16;
     int rayTileFirst;         // first ray tile of this scene
-    // per-tile ray-distance bounds (vm_dev_ray.h) of the scene's 8x8 pixel
+    // per-tile ray-distance bounds (vm_dev_ray.h) of the scene's pixel
     // tiles; the epoch of a step is the batch's step count + 1
     unsigned long long* rayKeys;
     int rayTiles;

@@ -337,7 +337,7 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
             return fail(VM_ERR_OUT_OF_MEMORY, msg);
         }
     }
-    // the raycast's tile bounds for images up to 2^16 8x8 tiles, so that a
+    // the raycast's tile bounds for images up to 2^16 bounds tiles, so that a
     // raycast of such an image allocates nothing
     RayBounds rb0;
     int rc = ray_bounds_for(m, 1, 1, nullptr, &rb0);

@@ -57,6 +57,10 @@ enum VoxelFormat : int {
     kFmtF32 = 2,     // float sdf, float wt, uchar rgb[3] + pad: 12 B, split per block (Vox<kFmtF32>)
 };
 
+// The raycast's ray-distance bounds are kept per 8x8 pixel tile, the rays of
+// one wave of the raycast.
+constexpr int kBoundsTileShift = 3;
+
 // Plain-old-data view of one map, passed to kernels by value.
 struct MapView {
     HashEntry* table;      // nEntry slots: nBucket heads, then the excess list

@@ -649,6 +649,13 @@ __host__ __device__ inline float ray_drift_margin(int W, int H, const Intrinsics
     return 1.0f + 1.75f * dEnd * (0.5f * ulp + 0x1p-23f) + 8.0f * ulp;
 }
 
+// Pixel tiles of the bounds (2^kBoundsTileShift pixels square, vm_common.h),
+// row-major over the image; a lane takes the tile of its own pixel
+__host__ __device__ inline int bounds_tiles_x(int W) { return (W + (1 << kBoundsTileShift) - 1) >> kBoundsTileShift; }
+__host__ __device__ inline int bounds_tile(int x, int y, int W) {
+    return (y >> kBoundsTileShift) * bounds_tiles_x(W) + (x >> kBoundsTileShift);
+}
+
 // [nearLim, farLim] of tile t for raycast_pixel; without keys (a pose whose
 // rotation is not orthonormal) the whole ray
 __device__ inline void ray_bounds_limits(const RayBounds& rb, int t, float margin, float& nearLim, float& farLim) {
@@ -793,7 +800,7 @@ template <int CH>
 __device__ __forceinline__ void ray_bounds_blocks(const int4 b, const RayBounds& rb, int W, int H, const Intrinsics& K,
                                          const Affine& Tinv, float3 c, float vs, float grow, BoundsShared<CH>& sh) {
     const int tid = threadIdx.x;
-    const int tilesX = (W + 7) >> 3;
+    const int tilesX = bounds_tiles_x(W);
     const unsigned long long ne = (unsigned long long)(~rb.epoch) << 32, fe = (unsigned long long)rb.epoch << 32;
     unsigned long long* nearK = rb.keys;
     unsigned long long* farK = rb.keys + rb.nTiles + 1;
@@ -806,7 +813,8 @@ __device__ __forceinline__ void ray_bounds_blocks(const int4 b, const RayBounds&
             atomicMin(&nearK[rb.nTiles], ne | nb);
             atomicMax(&farK[rb.nTiles], fe | fb);
         } else if (r.kind == kReachRect) {
-            const int tx0 = r.x0 >> 3, tx1 = r.x1 >> 3, ty0 = r.y0 >> 3, ty1 = r.y1 >> 3;
+            const int tx0 = r.x0 >> kBoundsTileShift, tx1 = r.x1 >> kBoundsTileShift;
+            const int ty0 = r.y0 >> kBoundsTileShift, ty1 = r.y1 >> kBoundsTileShift;
             const int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
             if (n <= kBoundsLightTiles) {
                 for (int ty = ty0; ty <= ty1; ++ty)

@@ -43,7 +43,10 @@ hipError_t launch_raycast(const MapView& m, float* vmap, int64_t pitch, int widt
                           const Intrinsics& K, const Affine& camToWorld, const Affine& worldToCam,
                           bool gate, hipStream_t s, const RayBounds* bounds = nullptr, bool boundsReady = false);
 // tiles of a width x height image (the bounds hold 2 (tiles + 1) 64-bit keys)
-inline int ray_bounds_tiles(int width, int height) { return ((width + 7) / 8) * ((height + 7) / 8); }
+inline int ray_bounds_tiles(int width, int height) {
+    const int t = 1 << kBoundsTileShift;
+    return ((width + t - 1) / t) * ((height + t - 1) / t);
+}
 // RenderingBlockDelegate + FillRenderingBlockFunctor, ray_tracing.cu:13-234
 hipError_t launch_rendering_blocks(const MapView& m, float* zx, float* zy, int64_t pitch,
                                    int cols8, int rows8, const Intrinsics& K,

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kRaycastWave
     if (x >= W || y >= H) return;
     if (gate && m.ctr[kVisibleCount] == 0) return;
     float nearLim = 0.0f, farLim = INFINITY;
-    if (rb.keys) ray_bounds_limits(rb, t, margin, nearLim, farLim);
+    if (rb.keys) ray_bounds_limits(rb, bounds_tile(x, y, W), margin, nearLim, farLim);
     float* out = (float*)((uint8_t*)vmap + (int64_t)y * pitch) + x * 4;
     raycast_pixel<FMT, kRaycastRuns>(m, out, x, y, K, pose, Tinv, vsInv, rtStep, nearLim, farLim);
 }
