@@ -104,8 +104,8 @@ __global__ __launch_bounds__(256) void kb_visible(BatchView B) {
         const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);
         const int i = base + (int)threadIdx.x;
         const int4 b = i < A ? S.m.blockPos[i] : make_int4(kEmptyKey, 0, 0, 0);
-        ray_bounds_blocks<256>(b, RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u}, S.d.width, S.d.height, S.K,
-                               S.posesInv[f], cc, S.m.voxelSize, grow, bsh);
+        ray_bounds_blocks<256, kBatchBoundsTileShift>(b, RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u}, S.d.width,
+                                                      S.d.height, S.K, S.posesInv[f], cc, S.m.voxelSize, grow, bsh);
     }
 }
 
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void kb
     const Affine& T = S.poses[f];
     const float margin = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
     float nearLim, farLim;
-    ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u}, bounds_tile(x, y, S.d.width),
+    ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u}, bounds_tile<kBatchBoundsTileShift>(x, y, S.d.width),
                       margin, nearLim, farLim);
     float* out = (float*)((uint8_t*)S.vmap + (int64_t)y * S.vpitch) + x * 4;
     raycast_pixel<FMT>(S.m, out, x, y, S.K, T, S.posesInv[f], S.vsInv, S.rtStep, nearLim, farLim);
@@ -281,8 +281,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPersistWav
         const float3 cc = make_float3(T.t0 * S.vsInv, T.t1 * S.vsInv, T.t2 * S.vsInv);
         const int i = base + (int)threadIdx.x;
         const int4 b = i < A ? S.m.blockPos[i] : make_int4(kEmptyKey, 0, 0, 0);
-        ray_bounds_blocks<256>(b, RayBounds{S.rayKeys, S.rayTiles, epoch}, S.d.width, S.d.height, S.K,
-                               S.posesInv[f], cc, S.m.voxelSize, grow, bsh);
+        ray_bounds_blocks<256, kBatchBoundsTileShift>(b, RayBounds{S.rayKeys, S.rayTiles, epoch}, S.d.width,
+                                                      S.d.height, S.K, S.posesInv[f], cc, S.m.voxelSize, grow, bsh);
     }
     grid_barrier(P, (4u * seq + 2u) * nWg);
 
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPersistWav
                 const float margin = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
                 float nearLim, farLim;
                 ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, epoch},
-                                  bounds_tile(x, y, S.d.width), margin, nearLim, farLim);
+                                  bounds_tile<kBatchBoundsTileShift>(x, y, S.d.width), margin, nearLim, farLim);
                 float* out = (float*)((uint8_t*)S.vmap + (int64_t)y * S.vpitch) + x * 4;
                 raycast_pixel<FMT>(S.m, out, x, y, S.K, T, S.posesInv[f], S.vsInv, S.rtStep, nearLim, farLim);
             }

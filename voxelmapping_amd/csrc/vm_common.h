@@ -58,8 +58,11 @@ enum VoxelFormat : int {
 };
 
 // The raycast's ray-distance bounds are kept per 8x8 pixel tile, the rays of
-// one wave of the raycast.
+// one wave of the raycast; the multi-scene step keeps them per 16x16 tile
+// (fewer tile writes in its bounds pass, which weigh more there than the
+// looser limits in its raycast).
 constexpr int kBoundsTileShift = 3;
+constexpr int kBatchBoundsTileShift = 4;
 
 // Plain-old-data view of one map, passed to kernels by value.
 struct MapView {

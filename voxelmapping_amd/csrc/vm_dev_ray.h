@@ -649,11 +649,14 @@ __host__ __device__ inline float ray_drift_margin(int W, int H, const Intrinsics
     return 1.0f + 1.75f * dEnd * (0.5f * ulp + 0x1p-23f) + 8.0f * ulp;
 }
 
-// Pixel tiles of the bounds (2^kBoundsTileShift pixels square, vm_common.h),
-// row-major over the image; a lane takes the tile of its own pixel
-__host__ __device__ inline int bounds_tiles_x(int W) { return (W + (1 << kBoundsTileShift) - 1) >> kBoundsTileShift; }
+// Pixel tiles of the bounds (2^SHIFT pixels square; kBoundsTileShift and
+// kBatchBoundsTileShift, vm_common.h), row-major over the image; a lane
+// takes the tile of its own pixel
+template <int SHIFT = kBoundsTileShift>
+__host__ __device__ inline int bounds_tiles_x(int W) { return (W + (1 << SHIFT) - 1) >> SHIFT; }
+template <int SHIFT = kBoundsTileShift>
 __host__ __device__ inline int bounds_tile(int x, int y, int W) {
-    return (y >> kBoundsTileShift) * bounds_tiles_x(W) + (x >> kBoundsTileShift);
+    return (y >> SHIFT) * bounds_tiles_x<SHIFT>(W) + (x >> SHIFT);
 }
 
 // [nearLim, farLim] of tile t for raycast_pixel; without keys (a pose whose
@@ -796,11 +799,11 @@ __device__ __forceinline__ Reach block_reach(const int4 b, int W, int H, const I
 // the workgroup's threads take their tiles in turn.
 // The same for blocks whose positions the caller has loaded: b is this
 // thread's block (x = kEmptyKey for none).
-template <int CH>
+template <int CH, int SHIFT = kBoundsTileShift>
 __device__ __forceinline__ void ray_bounds_blocks(const int4 b, const RayBounds& rb, int W, int H, const Intrinsics& K,
                                          const Affine& Tinv, float3 c, float vs, float grow, BoundsShared<CH>& sh) {
     const int tid = threadIdx.x;
-    const int tilesX = bounds_tiles_x(W);
+    const int tilesX = bounds_tiles_x<SHIFT>(W);
     const unsigned long long ne = (unsigned long long)(~rb.epoch) << 32, fe = (unsigned long long)rb.epoch << 32;
     unsigned long long* nearK = rb.keys;
     unsigned long long* farK = rb.keys + rb.nTiles + 1;
@@ -813,8 +816,8 @@ __device__ __forceinline__ void ray_bounds_blocks(const int4 b, const RayBounds&
             atomicMin(&nearK[rb.nTiles], ne | nb);
             atomicMax(&farK[rb.nTiles], fe | fb);
         } else if (r.kind == kReachRect) {
-            const int tx0 = r.x0 >> kBoundsTileShift, tx1 = r.x1 >> kBoundsTileShift;
-            const int ty0 = r.y0 >> kBoundsTileShift, ty1 = r.y1 >> kBoundsTileShift;
+            const int tx0 = r.x0 >> SHIFT, tx1 = r.x1 >> SHIFT;
+            const int ty0 = r.y0 >> SHIFT, ty1 = r.y1 >> SHIFT;
             const int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
             if (n <= kBoundsLightTiles) {
                 for (int ty = ty0; ty <= ty1; ++ty)
