@@ -160,8 +160,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void kb
     const Affine& T = S.poses[f];
     const float margin = ray_drift_margin(S.d.width, S.d.height, S.K, T.t0, T.t1, T.t2, S.vsInv);
     float nearLim, farLim;
-    ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u}, bounds_tile<kBatchBoundsTileShift>(x, y, S.d.width),
-                      margin, nearLim, farLim);
+    ray_bounds_limits(RayBounds{S.rayKeys, S.rayTiles, *B.steps + 1u},
+                      bounds_tile<kBatchBoundsTileShift>(x, y, S.d.width), margin, nearLim, farLim);
     float* out = (float*)((uint8_t*)S.vmap + (int64_t)y * S.vpitch) + x * 4;
     raycast_pixel<FMT>(S.m, out, x, y, S.K, T, S.posesInv[f], S.vsInv, S.rtStep, nearLim, farLim);
 }
