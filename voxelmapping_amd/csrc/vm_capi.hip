@@ -148,7 +148,7 @@ int ray_bounds_for(vm_map* m, int W, int H, hipStream_t s, RayBounds* out) {
             // raycasts already queued on any stream may still read the old keys
             if (m->rayKeys) {
                 VM_HIP(hipDeviceSynchronize(), "device synchronize");
-                hipFree(m->rayKeys);
+                VM_HIP(hipFree(m->rayKeys), "raycast bounds free");
             }
             m->rayKeys = nullptr;
             m->rayKeysTiles = 0;
@@ -316,7 +316,10 @@ int vm_map_create(const vm_map_desc* d, vm_map** out) {
     v.maxWeight = m->desc.max_weight;
     v.fmt = d->voxel_format;
     v.occMask = brick_words(v.nBlock) - 1;
-    hipGetDevice(&m->device);
+    if (hipError_t e = hipGetDevice(&m->device); e != hipSuccess) {
+        vm_map_destroy(m);
+        return hip_fail(e, "current device");
+    }
 
     const size_t vb = (size_t)voxel_bytes(v.fmt);
     struct { void** p; size_t bytes; const char* name; } allocs[] = {
@@ -361,8 +364,9 @@ int vm_map_destroy(vm_map* m) {
     MapView& v = m->view;
     void* ptrs[] = {v.table, v.visible, v.heap, v.mutex, v.voxels, v.blockPos, v.ctr, v.occ, m->scratch,
                     m->rayKeys, m->meshWork};
+    // teardown: a failed free leaves nothing for the caller to recover
     for (void* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     delete m;
     return VM_OK;
 }
@@ -977,8 +981,8 @@ struct vm_batch {
 namespace {
 
 void batch_drop_graph(vm_batch* b) {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
 }
@@ -1221,16 +1225,17 @@ int vm_batch_create(const vm_batch_scene* scenes, int32_t n, vm_batch** out) {
 int vm_batch_destroy(vm_batch* b) {
     if (!b) return VM_OK;
     batch_drop_graph(b);
+    // teardown: failures here leave nothing for the caller to recover
     for (hipStream_t t : b->side) {
-        hipStreamSynchronize(t);
-        hipStreamDestroy(t);
+        (void)hipStreamSynchronize(t);
+        (void)hipStreamDestroy(t);
     }
-    for (hipEvent_t ev : b->join) hipEventDestroy(ev);
-    if (b->fork) hipEventDestroy(b->fork);
-    if (b->dScenes) hipFree(b->dScenes);
-    if (b->dPoses) hipFree(b->dPoses);
-    if (b->dRayKeys) hipFree(b->dRayKeys);
-    if (b->dFrame) hipFree(b->dFrame);
+    for (hipEvent_t ev : b->join) (void)hipEventDestroy(ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->dScenes) (void)hipFree(b->dScenes);
+    if (b->dPoses) (void)hipFree(b->dPoses);
+    if (b->dRayKeys) (void)hipFree(b->dRayKeys);
+    if (b->dFrame) (void)hipFree(b->dFrame);
     delete b;
     return VM_OK;
 }
@@ -1279,7 +1284,7 @@ int vm_batch_capture(vm_batch* b, void* stream) {
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &g);
     if (e != hipSuccess) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return hip_fail(e, "captured step");
     }
     if (e2 != hipSuccess) return hip_fail(e2, "end capture");
@@ -1349,7 +1354,7 @@ int vm_batch_tune(vm_batch* b, void* stream, int32_t steps, vm_batch_tuning* bes
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
             return fail(VM_ERR_HIP, "event timing");
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return fail(VM_ERR_HIP, "event timing");
         if (ms < bestMs) { bestMs = ms; best = t; }
         return VM_OK;
     };
@@ -1373,8 +1378,8 @@ int vm_batch_tune(vm_batch* b, void* stream, int32_t steps, vm_batch_tuning* bes
     rc = vm_batch_set_tuning(b, &best);
     if (rc == VM_OK && best_out) *best_out = best;
 done:
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return rc;
 }
 
