@@ -162,8 +162,12 @@ __global__ __launch_bounds__(kPoseBoundsThreads) void k_pose_bounds(MapView m, i
     // projection is skipped
     float n2 = INFINITY, f2 = 0.0f;
     const int stride = gridDim.x * kPoseBoundsThreads;
-    // four blocks per pass, their positions loaded together
-    for (int i0 = blockIdx.x * kPoseBoundsThreads + threadIdx.x; i0 < A; i0 += 4 * stride) {
+    // four blocks per pass, their positions loaded together; the passes are
+    // counted per wave (from its first lane's block), so that every lane
+    // takes part in the shuffles below
+    const int wave0 = blockIdx.x * kPoseBoundsThreads + (threadIdx.x & ~63);
+    for (int p0 = 0; wave0 + p0 < A; p0 += 4 * stride) {
+        const int i0 = wave0 + p0 + (threadIdx.x & 63);
         int4 bq[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
